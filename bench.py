@@ -1,0 +1,174 @@
+"""Benchmark of the batched GSO pose-scoring path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--family dna] [--no-cpu]
+
+Workload (one step = one scoring launch of one GSO step of 6 swarms x 200
+glowworms): synthetic 12,625-atom receptor (RSA, 3.0 A separation, (60, 55, 50) A
+ellipsoid) x 1,500-atom ligand, 1,200 poses per step, dna flags (far field +
+electrostatics + clash), FP32 pair terms.  Poses for all steps are resident in
+HBM before the timed region; each step scores a fresh 1,200-pose slice into a
+device buffer.  Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "lklight-gpu_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "pose-evals/s (dna, 12.6k-atom rec); surrogate/plain speedup at equal top-10 E"
+HBM_PEAK_GBS = 8000.0
+POSES_PER_STEP = 1200
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def pose_bytes(nl, n_far, n_near, desolv=False, nr=0):
+    """Algorithmic bytes of one pose evaluation (no-reuse gather model):
+    pose 7 f64 + out 2 f64 + ligand 25 B/atom + 8 phi corners per charged atom
+    sampled in the grid + one 25 B receptor record per pair within 10 A."""
+    b = 56 + 16 + 25 * nl + 32 * n_far + 25 * n_near
+    if desolv:
+        b += 8 * (nr + nl)
+    return b
+
+
+def cpu_baseline(cx, poses, method, threads, budget_s=15.0):
+    """The oracle (C restatement of the reference CPU grid path, f64) on the
+    host cores, parallel over poses like the reference's rayon loop."""
+    from oracle import lkl_oracle as O
+    orc = O.Oracle(cx)
+    n0 = max(threads * 2, 32)
+    t = time.perf_counter()
+    r = orc.batch(method, poses[:n0], nthreads=threads, counts=True)
+    dt = time.perf_counter() - t
+    n = int(min(poses.shape[0], max(n0, n0 * budget_s / max(dt, 1e-3))))
+    t = time.perf_counter()
+    r = orc.batch(method, poses[:n], nthreads=threads, counts=True)
+    dt = time.perf_counter() - t
+    return n / dt, n, dt, r
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--family", default="dna", choices=["dna", "pydock", "vdw", "cpydock"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cell", type=float, default=0.0)
+    ap.add_argument("--block", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus %d is not supported "
+                          "(poses shard across GPUs with no exchange, but no multi-GPU path is "
+                          "built)" % args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import lklight
+    from lklight import synth
+
+    if not lklight.device_available():
+        raise SystemExit("no gfx950 device: the benchmark measures the HIP path only")
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+
+    log("generating 12,625-atom receptor / 1,500-atom ligand")
+    cx = synth.make_config("c2")
+    t = time.perf_counter()
+    cx.build_field()
+    log("phi %s built on GPU in %.2f s" % (cx.dims, time.perf_counter() - t))
+    desolv = args.family == "cpydock"
+    ctx = lklight.Context(cx, desolv=desolv)
+    if args.cell or args.block:
+        ctx.set_tiles(args.cell or 5.0, args.block or 256)
+    flags = lklight.family_flags(args.family) | (lklight.F_DESOLV if desolv else 0)
+
+    K, W, N = args.steps, args.warmup, POSES_PER_STEP
+    poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
+    d_poses = torch.from_numpy(poses).to(dev)
+    d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
+    d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+
+    def step(i):
+        ctx.score_async(d_poses[i * N:(i + 1) * N].data_ptr(), N, flags, d_out[i].data_ptr(),
+                        d_s[i].data_ptr() if desolv else None, sh)
+
+    for i in range(W):
+        step(i)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(K)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        ev[k][0].record(stream)
+        step(W + k)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
+    value = N * K / wall
+    log("timed %d steps: %.3f s wall, kernel avg %.3f ms" % (K, wall, avg_kernel_s * 1e3))
+
+    # CPU baseline (oracle) on a bounded sample of the same poses; its pair
+    # counts also give the algorithmic bytes of the roofline figure.
+    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+    cpu = None
+    n_near = n_far = None
+    sample_poses = poses[W * N:]
+    if not args.no_cpu:
+        log("cpu baseline: oracle on %d threads" % threads)
+        rate, n, dt, r = cpu_baseline(cx, sample_poses, args.family, threads)
+        n_near, n_far = float(r["n_near"].mean()), float(r["n_far"].mean())
+        g = d_out[W:].reshape(-1, 2)[:n].cpu().numpy()
+        sc_g = lklight.fold_scores(args.family, g[:, 0], g[:, 1],
+                                   d_s[W:].reshape(-1)[:n].cpu().numpy() if desolv else None)
+        sc_o = lklight.fold_scores(args.family, r["e"], r["v"], r["s"] if desolv else None)
+        max_rel = float(np.max(np.abs(sc_g - sc_o) / np.maximum(np.abs(sc_o), 1.0)))
+        model = "unknown"
+        try:
+            for ln in open("/proc/cpuinfo"):
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        cpu = {"value": rate, "unit": "pose-evals/s", "cores": threads, "kind": "port",
+               "sample": "first %d poses of the timed workload (%.1f s), %s, f64 C restatement "
+                         "of the reference CPU grid path, -O3 -march=x86-64" % (n, dt, model),
+               "gpu_over_cpu": value / rate, "max_rel_err_vs_cpu": max_rel}
+    b_pose = pose_bytes(cx.lig.n, n_far or 0.0, n_near or 0.0, desolv, cx.rec.n)
+    achieved = b_pose * N / avg_kernel_s / 1e9 if n_near is not None else None
+    out = {
+        "metric": METRIC, "value": value, "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": "c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "
+                               "(6 swarms x 200 glowworms), phi %s at 0.5 A"
+                               % (args.family, N, "x".join(map(str, cx.dims))),
+                   "family": args.family, "flags": flags, "poses_per_step": N},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel_ms": avg_kernel_s * 1e3, "bytes_per_pose": b_pose,
+                     "n_near_per_pose": n_near, "n_far_per_pose": n_far},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,160 @@
+/*
+ * lklight_gpu.h — C ABI of the MI355X (gfx950) batched GSO pose-scoring path.
+ *
+ * Drop-in boundary for LKlight's GPU bridge.  The four stateless entry points
+ * keep the argument lists, argument meaning and 0 / -1 status convention of the
+ * reference's extern "C" functions (new lkl_ names, no cuda* symbols).  The
+ * context API underneath them is explicit: create (upload constants once),
+ * score (per GSO step), destroy.
+ *
+ * Conventions (all entry points):
+ *   - the caller owns every buffer passed in; nothing is retained after return
+ *     except device copies held by a context;
+ *   - return 0 on success, -1 on failure; lkl_last_error() returns the
+ *     message of the calling thread's last failure (also printed to stderr);
+ *   - results are deterministic: per-pose sums are reduced in a fixed order,
+ *     independent of launch shape and scheduling;
+ *   - N (poses per call) is bounded only by memory (no 65,535 limit);
+ *   - one context must not be used by two threads at once; distinct contexts
+ *     are independent.  The stateless wrappers serialise on an internal mutex.
+ *
+ * Poses are N x 7 f64 rows (w, x, y, z, tx, ty, tz): unit quaternion then
+ * translation, as packed by gpu_score.rs:292-301.  Score outputs are N x 2 f64
+ * rows (E_raw, V): raw electrostatics (x 332/4 for kcal/mol) and the capped
+ * Lennard-Jones sum (+ heavy-atom clash when F_CLASH is set).
+ */
+#ifndef LKLIGHT_GPU_H
+#define LKLIGHT_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Family flag word (full_score.cu:23-25, gpu_score.rs:15-21).  LJ is always on. */
+#define LKL_F_FAR 1u    /* 10-30 A far field  q_j * phi(x_j)          */
+#define LKL_F_ELEC 2u   /* <= 10 A clamped electrostatics             */
+#define LKL_F_CLASH 4u  /* heavy-atom linear clash penalty            */
+#define LKL_F_DESOLV 16u /* cpydock contact-SASA desolvation (S)       */
+#define LKL_FLAGS_DNA 7u
+#define LKL_FLAGS_PYDOCK 3u
+#define LKL_FLAGS_VDW 0u
+
+/* ── stateless entry points (reference signatures) ─────────────────────────── */
+
+/* Replaces cuda_batch_score (full_score.cu:359-371), called from
+ * gpu_score.rs:252-261 and gpu_family.rs:211-221.  The receptor cell list
+ * arguments are accepted for signature compatibility; the library builds its
+ * own cell-sorted layout from r_coords.  out must hold N*2 doubles. */
+int lkl_batch_score(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                    float sp, const float* r_coords, const float* r_ele, const float* r_svdw,
+                    const float* r_vdwr, const unsigned char* r_heavy, int nr,
+                    const int* cell_start, const int* cell_atoms, int ncx, int ncy, int ncz,
+                    float c_ox, float c_oy, float c_oz, float c_sp, const float* l_base,
+                    const double* poses, const float* l_ele, const float* l_svdw,
+                    const float* l_vdwr, const unsigned char* l_heavy, int nl, int N,
+                    unsigned flags, double* out);
+
+/* Replaces cuda_cpydock_solv (full_score.cu:571-581), called from
+ * gpu_family.rs:360-372.  outS[k] = desolvation S of pose k, the term of
+ * cpydock's score = -(E*332/4 + 0.1*V - S).  l_ele is unused (as in the
+ * reference). */
+int lkl_cpydock_solv(const float* r_coords, const unsigned char* r_flag, int nr,
+                     const int* cell_start, const int* cell_atoms, int ncx, int ncy, int ncz,
+                     float c_ox, float c_oy, float c_oz, float c_sp, const float* l_base,
+                     const double* poses, const unsigned char* l_flag, const float* l_ele,
+                     int nl, int N, const float* r_des, const float* r_asa,
+                     const float* l_des, const float* l_asa, double* outS);
+
+/* Replaces cuda_full_score (full_score.cu:140-150), called from
+ * gpu_score.rs:159-168: one pre-transformed pose, per-ligand-atom outputs
+ * (out_elec[j] raw electrostatics near + far, out_vdw[j] LJ + clash), with
+ * the dna flag word (far + elec + clash). */
+int lkl_full_score(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                   float sp, const float* r_coords, const float* r_ele, const float* r_svdw,
+                   const float* r_vdwr, const unsigned char* r_heavy, int nr,
+                   const int* cell_start, const int* cell_atoms, int ncx, int ncy, int ncz,
+                   float c_ox, float c_oy, float c_oz, float c_sp, const float* l_coords,
+                   const float* l_ele, const float* l_svdw, const float* l_vdwr,
+                   const unsigned char* l_heavy, int nl, float* out_elec, float* out_vdw);
+
+/* Replaces cuda_far_field (far_field.cu:59-63), called from gpu_field.rs:20-35:
+ * *result = sum_j q_j * phi(x_j) over pre-transformed coordinates. */
+int lkl_far_field(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                  float sp, const float* coords, const float* charges, int n, double* result);
+
+/* Replaces the direct cudaGetDeviceCount probe of gpu_score.rs:226-232
+ * (cuda_available) and gpu_family.rs:322-337: 1 if a usable gfx950 device is
+ * present, else 0.  Never fails. */
+int lkl_device_available(void);
+
+/* Message of the calling thread's last failed call ("" if none).  No
+ * reference counterpart: the reference prints to stderr only
+ * (full_score.cu:463). */
+const char* lkl_last_error(void);
+
+/* Drop every context cached by the stateless entry points (their device copies
+ * are keyed on host-pointer identity, as in full_score.cu:380-387).  Call
+ * after mutating a cached host buffer in place.  No reference counterpart. */
+void lkl_cache_clear(void);
+
+/* ── context API (no reference counterpart; the stateless entry points above
+ *    are implemented on it) ──────────────────────────────────────────────── */
+
+typedef struct lkl_ctx lkl_ctx;
+
+/* One complex: receptor far field, receptor and ligand atoms (reference
+ * order), optional cpydock desolvation sides.  Unused pointers may be NULL
+ * (phi == NULL or nx == 0 means "no far field": F_FAR is then rejected;
+ * r_flag == NULL means "no desolvation": F_DESOLV is then rejected). */
+typedef struct {
+    const float* phi;  /* nx*ny*nz, x fastest (grid_dna.rs:151-154) */
+    int nx, ny, nz;
+    double ox, oy, oz, sp;
+    int nr;
+    const float* r_xyz; /* nr*3 */
+    const float* r_ele;
+    const float* r_svdw; /* sqrt(epsilon) */
+    const float* r_vdwr;
+    const unsigned char* r_heavy;
+    int nl;
+    const float* l_xyz; /* nl*3, reference (untransformed) coordinates */
+    const float* l_ele;
+    const float* l_svdw;
+    const float* l_vdwr;
+    const unsigned char* l_heavy;
+    const unsigned char* r_flag; /* cpydock exclusion masks (cpydock.rs:808-819) */
+    const float* r_des;
+    const float* r_asa;
+    const unsigned char* l_flag;
+    const float* l_des;
+    const float* l_asa;
+    float cell_size; /* receptor cell edge in A; 0 = library default */
+} lkl_complex_desc;
+
+/* Upload one complex.  Returns NULL on failure (see lkl_last_error). */
+lkl_ctx* lkl_ctx_create(const lkl_complex_desc* desc);
+void lkl_ctx_destroy(lkl_ctx* ctx);
+
+/* Blocking, host buffers: poses N*7, out N*2 (E_raw, V), outS N (needed only
+ * with LKL_F_DESOLV, may be NULL otherwise). */
+int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, double* out,
+                  double* outS);
+
+/* Asynchronous, device buffers, enqueued on `stream` (a hipStream_t; NULL =
+ * the context's own stream).  d_out / d_outS as above; d_outS may be NULL
+ * without LKL_F_DESOLV.  Launches exactly one kernel. */
+int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
+                        double* d_out, double* d_outS, void* stream);
+
+/* Diagnostic: f64 transformed ligand coordinates of N poses (N*nl*3, ligand
+ * order of the descriptor), computed by the device transform. */
+int lkl_ctx_transform(lkl_ctx* ctx, const double* poses, int N, double* out_xyz);
+
+/* Set the receptor cell edge and workgroup size used by later score calls
+ * (autotuner hook).  cell_size in A (> 0), block in {64,128,256,512}. */
+int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

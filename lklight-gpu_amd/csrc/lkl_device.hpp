@@ -1,0 +1,181 @@
+// lkl_device.hpp — device-side building blocks of the gfx950 pose-scoring path.
+//
+// Numerics contract (what makes the FP32 kernel agree with the f64 CPU grid
+// path of dna.rs:834-998 / pydock.rs:353-447 / vdw.rs:73-151 /
+// cpydock.rs:556-676 to ~1e-7 relative):
+//   * the pose transform is f64 with contraction disabled, evaluated in the
+//     order of qt.rs:67-74 + qt.rs:122-128 + dna.rs:868-873, so the
+//     transformed coordinates equal the CPU's bit for bit;
+//   * pair distances are filtered in f32 from a hi/lo split of the f64
+//     ligand coordinate (dx = (x_hi - r_x) + x_lo, relative error ~1e-7);
+//   * pairs within 1e-3 A^2 of the 10 A cutoff, and every pair entering the
+//     6.4 A desolvation window, are re-measured in exact f64 (same expression
+//     order as the CPU), so cutoff and window membership match the CPU exactly;
+//   * per-pair terms are f32, accumulated as int64 fixed point (E and S at
+//     2^-37, V at 2^-26): integer sums are associative, so per-pose results
+//     are bitwise identical for every launch shape, cell size, pose batching
+//     and multi-problem packing; the per-term rounding (<= 7e-12 for E,
+//     <= 7.5e-9 for V) is far below the f32 term error;
+//   * the far field is sampled in f64 exactly like grid_dna.rs:159-197
+//     (fractional offsets rounded to f32), from the same f32 phi values.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lkl {
+
+// Constants of full_score.cu:15-19 / dna.rs:20-42 / cpydock.rs:36-45.
+constexpr double ES_CAP = 1.0 * 4.0 / 332.0;  // clamp of the raw electrostatics
+constexpr float ES_CAPF = (float)ES_CAP;
+constexpr float LJ_CAP = 1.0f;
+constexpr float CP_W = 6.0f;
+constexpr float CP_F = 0.75f;
+constexpr double NEAR2 = 100.0;        // (10 A)^2, inclusive
+constexpr float NEAR_HI = 100.001f;    // f32 filter: pairs above never count
+constexpr float NEAR_LO = 99.999f;     // f32 filter: pairs above are re-measured in f64
+constexpr double SOLV2 = 6.4 * 6.4;    // 40.96000000000001 (cpydock.rs:44)
+constexpr float SOLV_HI = 40.97f;      // f32 filter of the desolvation window
+constexpr float WIN2 = 100.5f;         // cell window radius^2 (10.025 A, > 10 A + rounding)
+constexpr float SOLV_KEY_T = 40.96f;   // largest float <= SOLV2
+constexpr double MIN_INIT = 10000.0;   // HUGE_DISTANCE of cpydock.rs:592
+
+// Family flag word (full_score.cu:23-25) + desolvation bit.
+constexpr unsigned F_FAR = 1u, F_ELEC = 2u, F_CLASH = 4u, F_DESOLV = 16u;
+
+// One complex as seen by the kernels (passed by value).
+struct DevComplex {
+    // receptor, sorted by cell (x fastest, then y, then z)
+    const float4* rxyzq;  // x, y, z, q
+    const float4* rpar;   // sqrt(eps), vdw radius, heavy (0/1), 0
+    const int* cstart;    // ncx*ncy*ncz + 1 prefix offsets into the sorted atoms
+    const unsigned char* rflag;
+    const float* rdes;
+    const float* rasa;
+    int nr, ncx, ncy, ncz;
+    double cox, coy, coz, inv_cs;  // cell-grid origin and 1/edge
+    float rwin, rwin2;              // window radius in cells and its square
+    // ligand, spatially sorted (Morton order of the reference coordinates)
+    const float4* lxyzq;
+    const float4* lpar;
+    const unsigned char* lflag;
+    const float* ldes;
+    const float* lasa;
+    int nl;
+    // far field (grid_dna.rs:33-40), x fastest
+    const float* phi;
+    int nx, ny, nz;
+    double ox, oy, oz, sp;
+};
+
+struct Mat {
+    double m00, m01, m02, m10, m11, m12, m20, m21, m22, tx, ty, tz;
+};
+
+// qt.rs:67-74 — rotation matrix of a unit quaternion, row-major.
+__device__ __forceinline__ Mat pose_matrix(const double* __restrict__ p) {
+#pragma clang fp contract(off)
+    const double w = p[0], x = p[1], y = p[2], z = p[3];
+    Mat M;
+    M.m00 = 1. - 2. * (y * y + z * z);
+    M.m01 = 2. * (x * y - w * z);
+    M.m02 = 2. * (x * z + w * y);
+    M.m10 = 2. * (x * y + w * z);
+    M.m11 = 1. - 2. * (x * x + z * z);
+    M.m12 = 2. * (y * z - w * x);
+    M.m20 = 2. * (x * z - w * y);
+    M.m21 = 2. * (y * z + w * x);
+    M.m22 = 1. - 2. * (x * x + y * y);
+    M.tx = p[4];
+    M.ty = p[5];
+    M.tz = p[6];
+    return M;
+}
+
+// qt.rs:122-128 then + translation (dna.rs:868-873).
+__device__ __forceinline__ void transform(const Mat& M, float vxf, float vyf, float vzf,
+                                          double& x, double& y, double& z) {
+#pragma clang fp contract(off)
+    const double vx = vxf, vy = vyf, vz = vzf;
+    x = M.m00 * vx + M.m01 * vy + M.m02 * vz + M.tx;
+    y = M.m10 * vx + M.m11 * vy + M.m12 * vz + M.ty;
+    z = M.m20 * vx + M.m21 * vy + M.m22 * vz + M.tz;
+}
+
+// f64 squared distance in the CPU's expression order (dna.rs:940-943).
+__device__ __forceinline__ double exact_d2(double x, double y, double z, float4 r) {
+#pragma clang fp contract(off)
+    const double dx = x - (double)r.x;
+    const double dy = y - (double)r.y;
+    const double dz = z - (double)r.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// grid_dna.rs:159-197 — f64 trilinear interpolation, offsets rounded to f32.
+__device__ __forceinline__ double sample_phi(const DevComplex& C, double x, double y, double z) {
+#pragma clang fp contract(off)
+    const double fx = (x - C.ox) / C.sp;
+    const double fy = (y - C.oy) / C.sp;
+    const double fz = (z - C.oz) / C.sp;
+    const double ix = floor(fx), iy = floor(fy), iz = floor(fz);
+    if (ix < 0.0 || iy < 0.0 || iz < 0.0) return 0.0;
+    if (ix + 1.0 >= (double)C.nx || iy + 1.0 >= (double)C.ny || iz + 1.0 >= (double)C.nz)
+        return 0.0;
+    const double tx = (double)(float)(fx - ix);
+    const double ty = (double)(float)(fy - iy);
+    const double tz = (double)(float)(fz - iz);
+    const size_t sy = (size_t)C.nx, sz = (size_t)C.nx * (size_t)C.ny;
+    const size_t b = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+    const float* __restrict__ g = C.phi;
+    const double c000 = g[b], c100 = g[b + 1];
+    const double c010 = g[b + sy], c110 = g[b + sy + 1];
+    const double c001 = g[b + sz], c101 = g[b + sz + 1];
+    const double c011 = g[b + sz + sy], c111 = g[b + sz + sy + 1];
+    const double c00 = c000 * (1.0 - tx) + c100 * tx;
+    const double c10 = c010 * (1.0 - tx) + c110 * tx;
+    const double c01 = c001 * (1.0 - tx) + c101 * tx;
+    const double c11 = c011 * (1.0 - tx) + c111 * tx;
+    const double c0 = c00 * (1.0 - ty) + c10 * ty;
+    const double c1 = c01 * (1.0 - ty) + c11 * ty;
+    return c0 * (1.0 - tz) + c1 * tz;
+}
+
+// Monotone f32 key of an f64 squared distance that keeps membership of the
+// desolvation window exact: key <= bits(40.96f) iff d2 <= 6.4*6.4 (f64).
+// Non-negative floats order like their bit patterns, so ds_min_u32 on keys
+// is a min on distances.
+__device__ __forceinline__ unsigned solv_key(double d2) {
+    float f = (float)d2;
+    if (d2 <= SOLV2) f = fminf(f, SOLV_KEY_T);
+    else f = fmaxf(f, __uint_as_float(__float_as_uint(SOLV_KEY_T) + 1u));
+    return __float_as_uint(f);
+}
+
+// cpydock.rs:648-665 — desolvation term of one atom, clamp placed as on the CPU.
+__device__ __forceinline__ double solv_term(double m, float asaf, float desf) {
+    const double asa = asaf;
+    double solv = 0.0;
+    if (m <= SOLV2 && m > 0.0 && asa > 0.0) solv = -10.0 * sqrt(m) + 65.0;
+    if (solv > asa) solv = asa;
+    return solv * (double)desf;
+}
+
+// Fixed-point scales of the per-pose accumulators.
+constexpr double E_SCALE = 137438953472.0;  // 2^37: raw electrostatics, desolvation
+constexpr double V_SCALE = 67108864.0;      // 2^26: LJ + clash
+constexpr float E_SCALE_F = 137438953472.0f;
+constexpr float V_SCALE_F = 67108864.0f;
+constexpr float ES_CAP_SF = (float)(ES_CAP * E_SCALE);
+constexpr long long ES_CAP_I = 1655891006LL;  // round(4/332 * 2^37)
+
+__device__ __forceinline__ long long to_fix(double x, double scale) {
+    return __double2ll_rn(x * scale);
+}
+
+// Butterfly sum over a 64-lane wavefront (integer: exact, order-free).
+__device__ __forceinline__ long long wave_sum(long long x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+}  // namespace lkl

@@ -1,0 +1,819 @@
+// lkl_score.hip — gfx950 kernels + C ABI of the batched GSO pose-scoring path.
+//
+// One workgroup scores one pose: its lanes walk the (spatially sorted) ligand
+// atoms, transform each one in f64 (device twin of qt.rs:67-74,122-128), gather
+// the far field, and scan the receptor cell rows within 10 A.  Receptor atoms
+// are stored cell-sorted as packed float4 records, so a cell row (x-adjacent
+// cells) is one contiguous run and every candidate is one 16-byte load.  The
+// per-pose sums are int64 fixed point, reduced by wave butterflies and then
+// across waves — exact and order-free, replacing the per-block double atomics
+// of full_score.cu:332-335.
+//
+// cpydock's two-stage desolvation (full_score.cu:477-569: global atomicMin
+// scratch, 0x7F memset, second launch) is fused into the same pass: the
+// receptor-side minima live in LDS (ds_min_u32 on monotone f32 keys) and are
+// reduced by the same workgroup after a barrier.  Receptors too large for LDS
+// use a per-pose global scratch slice instead.
+#include "lkl_device.hpp"
+#include "../../include/lklight_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+using namespace lkl;
+
+#define LKL_API extern "C" __attribute__((visibility("default")))
+
+// ── error reporting + fault injection ─────────────────────────────────────────
+static thread_local std::string t_err;
+
+static void set_err(const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    t_err = buf;
+    fprintf(stderr, "[lklight-gpu] %s\n", buf);
+}
+
+// LKL_FAIL_AT=<stage> makes the named stage of every call fail, so host-only
+// tests can drive each failure path (stages: probe, alloc, upload, launch,
+// sync, readback).
+static bool fail_at(const char* stage) {
+    const char* e = getenv("LKL_FAIL_AT");
+    return e && strcmp(e, stage) == 0;
+}
+
+#define HIPCK(stage, expr)                                                            \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess || fail_at(stage)) {                                     \
+            set_err("%s stage=%s: %s", __func__, stage,                               \
+                    _e != hipSuccess ? hipGetErrorString(_e) : "injected failure");   \
+            return -1;                                                                \
+        }                                                                             \
+    } while (0)
+
+// ── kernels ───────────────────────────────────────────────────────────────────
+enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
+
+// Near field of one ligand atom: scan the receptor cell rows whose cells meet
+// the 10.025 A ball, accumulate the pair terms of full_score.cu:296-315 /
+// dna.rs:951-969 and (with desolvation) the per-atom minima of
+// cpydock.rs:620-627.
+template <bool ELEC, bool CLASH, int DES>
+__device__ __forceinline__ void near_atom(const DevComplex& C, double x64, double y64, double z64,
+                                          float qj, float svj, float rj, bool hj, bool lf,
+                                          long long& e, long long& v, double& minL,
+                                          unsigned* __restrict__ minR) {
+    // terms are formed directly in fixed-point units (power-of-two scales are exact)
+    const float qjs = qj * E_SCALE_F;
+    const float svjs = svj * V_SCALE_F;
+    const float xh = (float)x64, yh = (float)y64, zh = (float)z64;
+    const float xl = (float)(x64 - (double)xh);
+    const float yl = (float)(y64 - (double)yh);
+    const float zl = (float)(z64 - (double)zh);
+    // position in cell units (window arithmetic only; margins cover f32 error)
+    const float fx = fminf(fmaxf((float)((x64 - C.cox) * C.inv_cs), -1.0e4f), 1.0e4f);
+    const float fy = fminf(fmaxf((float)((y64 - C.coy) * C.inv_cs), -1.0e4f), 1.0e4f);
+    const float fz = fminf(fmaxf((float)((z64 - C.coz) * C.inv_cs), -1.0e4f), 1.0e4f);
+    const float rw = C.rwin, rw2 = C.rwin2;
+    const int izlo = max(0, (int)floorf(fz - rw)), izhi = min(C.ncz - 1, (int)floorf(fz + rw));
+    const int iylo = max(0, (int)floorf(fy - rw)), iyhi = min(C.ncy - 1, (int)floorf(fy + rw));
+    for (int iz = izlo; iz <= izhi; ++iz) {
+        const float gz = fmaxf(0.0f, fmaxf((float)iz - fz, fz - (float)(iz + 1)));
+        const float gz2 = gz * gz;
+        for (int iy = iylo; iy <= iyhi; ++iy) {
+            const float gy = fmaxf(0.0f, fmaxf((float)iy - fy, fy - (float)(iy + 1)));
+            const float g2 = gz2 + gy * gy;
+            if (g2 > rw2) continue;
+            const float b = sqrtf(rw2 - g2);
+            const int xlo = max(0, (int)floorf(fx - b));
+            const int xhi = min(C.ncx - 1, (int)floorf(fx + b));
+            if (xlo > xhi) continue;
+            const int row = (iz * C.ncy + iy) * C.ncx;
+            int p = C.cstart[row + xlo];
+            const int pe = C.cstart[row + xhi + 1];
+            for (; p < pe; ++p) {
+                const float4 r = C.rxyzq[p];
+                const float dx = (xh - r.x) + xl;
+                const float dy = (yh - r.y) + yl;
+                const float dz = (zh - r.z) + zl;
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
+                    if (C.rflag[p] == 0) {
+                        const double d2e = exact_d2(x64, y64, z64, r);
+                        minL = fmin(minL, d2e);
+                        atomicMin(&minR[p], solv_key(d2e));
+                    }
+                }
+                if (d2 <= NEAR_HI) {
+                    if (d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2) continue;
+                    const float4 rp = C.rpar[p];
+                    const float inv = __builtin_amdgcn_rcpf(d2);
+                    if (ELEC) {
+                        const float ae = qjs * r.w * inv;
+                        e += ae > ES_CAP_SF ? ES_CAP_I
+                                            : (ae < -ES_CAP_SF ? -ES_CAP_I : (long long)__float2int_rn(ae));
+                    }
+                    const float svs = svjs * rp.x;
+                    const float rr = rj + rp.y;
+                    const float rr2 = rr * rr;
+                    const float i2 = inv * inv;
+                    const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
+                    float vp = svs * (p6 * p6 - 2.0f * p6);
+                    vp = fminf(vp, LJ_CAP * V_SCALE_F);
+                    if (CLASH && hj && rp.z != 0.0f) {
+                        const float d = sqrtf(d2);
+                        const float dmin = CP_F * rr;
+                        if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+                    }
+                    v += (long long)__float2int_rn(vp);
+                }
+            }
+        }
+    }
+}
+
+// One workgroup per pose.  out[2k] = E_raw, out[2k+1] = V, outS[k] = S.
+// PERATOM additionally writes each ligand atom's (E, V) (sorted order) to pa_e/pa_v.
+template <bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
+__global__ void __launch_bounds__(512)
+score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
+             double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
+             float* __restrict__ pa_v) {
+    extern __shared__ unsigned lds_min[];
+    __shared__ long long red[3][16];
+    const int pose = blockIdx.x;
+    if (pose >= N) return;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const Mat M = pose_matrix(poses + (size_t)pose * 7);
+
+    unsigned* minR = nullptr;
+    if (DES == DES_LDS) minR = lds_min;
+    if (DES == DES_GLOBAL) minR = gmin + (size_t)pose * (size_t)C.nr;
+    if (DES != DES_NONE) {
+        const unsigned init = __float_as_uint((float)MIN_INIT);
+        for (int i = tid; i < C.nr; i += bs) minR[i] = init;
+        if (DES == DES_GLOBAL) __threadfence();
+        __syncthreads();
+    }
+
+    long long e = 0, v = 0, s = 0;
+    for (int j = tid; j < C.nl; j += bs) {
+        const float4 lb = C.lxyzq[j];
+        const float4 lp = C.lpar[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        long long ea = 0, va = 0;
+        double minL = MIN_INIT;
+        if (FAR && lb.w != 0.0f) ea += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
+        const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
+        near_atom<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, ea, va, minL,
+                                    minR);
+        if (PERATOM) {
+            pa_e[(size_t)pose * C.nl + j] = (float)((double)ea / E_SCALE);
+            pa_v[(size_t)pose * C.nl + j] = (float)((double)va / V_SCALE);
+        }
+        e += ea;
+        v += va;
+        if (DES != DES_NONE) s += to_fix(solv_term(minL, C.lasa[j], C.ldes[j]), E_SCALE);
+    }
+    if (DES != DES_NONE) {
+        if (DES == DES_GLOBAL) __threadfence();
+        __syncthreads();
+        for (int p = tid; p < C.nr; p += bs) {
+            unsigned k;
+            if (DES == DES_GLOBAL)
+                k = __hip_atomic_load(&minR[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            else
+                k = minR[p];
+            s += to_fix(solv_term((double)__uint_as_float(k), C.rasa[p], C.rdes[p]), E_SCALE);
+        }
+    }
+    e = wave_sum(e);
+    v = wave_sum(v);
+    if (DES != DES_NONE) s = wave_sum(s);
+    const int w = tid >> 6, nw = (bs + 63) >> 6;
+    if ((tid & 63) == 0) {
+        red[0][w] = e;
+        red[1][w] = v;
+        red[2][w] = s;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        long long te = 0, tv = 0, ts = 0;
+        for (int k = 0; k < nw; ++k) {
+            te += red[0][k];
+            tv += red[1][k];
+            ts += red[2][k];
+        }
+        out[2 * (size_t)pose] = (double)te / E_SCALE;
+        out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
+        if (DES != DES_NONE) outS[pose] = (double)ts / E_SCALE;
+    }
+}
+
+// Diagnostic: device transform of every (pose, ligand atom), sorted order.
+__global__ void transform_kernel(DevComplex C, const double* __restrict__ poses, int N,
+                                 double* __restrict__ outxyz) {
+    const int pose = blockIdx.x;
+    if (pose >= N) return;
+    const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    for (int j = threadIdx.x; j < C.nl; j += blockDim.x) {
+        const float4 lb = C.lxyzq[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        double* o = outxyz + ((size_t)pose * C.nl + j) * 3;
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+    }
+}
+
+// ── context ───────────────────────────────────────────────────────────────────
+struct lkl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t lds_max = 65536;
+    int block = 256;
+    float cell_size = 5.0f;
+    // host copies in reference order (kept to rebuild the cell layout)
+    int nr = 0, nl = 0;
+    std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
+    std::vector<unsigned char> r_heavy, r_flag;
+    bool has_phi = false, has_desolv = false;
+    // device
+    float* d_phi = nullptr;
+    int nx = 0, ny = 0, nz = 0;
+    double ox = 0, oy = 0, oz = 0, sp = 1;
+    float4 *d_rxyzq = nullptr, *d_rpar = nullptr;
+    int* d_cstart = nullptr;
+    unsigned char* d_rflag = nullptr;
+    float *d_rdes = nullptr, *d_rasa = nullptr;
+    float4 *d_lxyzq = nullptr, *d_lpar = nullptr;
+    unsigned char* d_lflag = nullptr;
+    float *d_ldes = nullptr, *d_lasa = nullptr;
+    std::vector<int> lperm;  // sorted position -> reference index
+    int ncx = 1, ncy = 1, ncz = 1;
+    double cox = 0, coy = 0, coz = 0;
+    // per-call buffers (grown on demand, never shrunk)
+    double* d_poses = nullptr;
+    size_t cap_poses = 0;
+    double* d_out = nullptr;
+    size_t cap_out = 0;
+    double* d_outS = nullptr;
+    size_t cap_outS = 0;
+    unsigned* d_gmin = nullptr;
+    size_t cap_gmin = 0;
+    float *d_pa_e = nullptr, *d_pa_v = nullptr;
+    size_t cap_pa = 0, cap_pa_v = 0;
+    double* d_xyz = nullptr;
+    size_t cap_xyz = 0;
+};
+
+template <typename T>
+static int dev_upload(T** dst, const T* src, size_t n) {
+    if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
+    if (n == 0) return 0;
+    HIPCK("alloc", hipMalloc((void**)dst, n * sizeof(T)));
+    HIPCK("upload", hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+template <typename T>
+static int dev_reserve(T** buf, size_t* cap, size_t n) {
+    if (n <= *cap && *buf) return 0;
+    if (*buf) { (void)hipFree(*buf); *buf = nullptr; *cap = 0; }
+    HIPCK("alloc", hipMalloc((void**)buf, std::max<size_t>(n, 1) * sizeof(T)));
+    *cap = n;
+    return 0;
+}
+
+// Receptor layout: uniform cells of edge cell_size over the receptor's box,
+// atoms stably counting-sorted by cell (x fastest), packed float4 records.
+static int build_receptor(lkl_ctx* c) {
+    const int nr = c->nr;
+    const double cs = c->cell_size;
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    if (nr > 0) {
+        for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
+        for (int i = 0; i < nr; ++i)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], (double)c->r_xyz[3 * i + a]);
+                hi[a] = std::max(hi[a], (double)c->r_xyz[3 * i + a]);
+            }
+    }
+    int n[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = std::floor(lo[a]) - 1.0;
+        n[a] = std::max(1, (int)std::floor((hi[a] - lo[a]) / cs) + 1);
+    }
+    const size_t ncell = (size_t)n[0] * n[1] * n[2];
+    if (ncell > (size_t)1 << 28) { set_err("receptor cell grid too large (%zu cells)", ncell); return -1; }
+    std::vector<int> cell(nr), start(ncell + 1, 0);
+    for (int i = 0; i < nr; ++i) {
+        int ii[3];
+        for (int a = 0; a < 3; ++a) {
+            ii[a] = (int)std::floor(((double)c->r_xyz[3 * i + a] - lo[a]) / cs);
+            ii[a] = std::min(std::max(ii[a], 0), n[a] - 1);
+        }
+        cell[i] = (ii[2] * n[1] + ii[1]) * n[0] + ii[0];
+        start[cell[i] + 1]++;
+    }
+    for (size_t k = 0; k < ncell; ++k) start[k + 1] += start[k];
+    std::vector<int> cur(start.begin(), start.end() - 1), order(nr);
+    for (int i = 0; i < nr; ++i) order[cur[cell[i]]++] = i;
+    std::vector<float4> xyzq(nr), par(nr);
+    std::vector<unsigned char> flag(nr, 0);
+    std::vector<float> des(nr, 0.f), asa(nr, 0.f);
+    for (int p = 0; p < nr; ++p) {
+        const int i = order[p];
+        xyzq[p] = make_float4(c->r_xyz[3 * i], c->r_xyz[3 * i + 1], c->r_xyz[3 * i + 2],
+                              c->r_ele.empty() ? 0.f : c->r_ele[i]);
+        par[p] = make_float4(c->r_svdw.empty() ? 0.f : c->r_svdw[i],
+                             c->r_vdwr.empty() ? 0.f : c->r_vdwr[i],
+                             (!c->r_heavy.empty() && c->r_heavy[i]) ? 1.f : 0.f, 0.f);
+        if (c->has_desolv) {
+            flag[p] = c->r_flag[i];
+            des[p] = c->r_des[i];
+            asa[p] = c->r_asa[i];
+        }
+    }
+    if (dev_upload(&c->d_rxyzq, xyzq.data(), (size_t)nr)) return -1;
+    if (dev_upload(&c->d_rpar, par.data(), (size_t)nr)) return -1;
+    if (dev_upload(&c->d_cstart, start.data(), ncell + 1)) return -1;
+    if (c->has_desolv) {
+        if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr)) return -1;
+        if (dev_upload(&c->d_rdes, des.data(), (size_t)nr)) return -1;
+        if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr)) return -1;
+    }
+    c->ncx = n[0];
+    c->ncy = n[1];
+    c->ncz = n[2];
+    c->cox = lo[0];
+    c->coy = lo[1];
+    c->coz = lo[2];
+    return 0;
+}
+
+// Morton code of a 10-bit-per-axis quantised point.
+static uint32_t spread3(uint32_t v) {
+    v &= 0x3ff;
+    v = (v | (v << 16)) & 0x30000ff;
+    v = (v | (v << 8)) & 0x300f00f;
+    v = (v | (v << 4)) & 0x30c30c3;
+    v = (v | (v << 2)) & 0x9249249;
+    return v;
+}
+
+// Ligand layout: Morton order of the reference coordinates (1 A quanta), so a
+// wavefront's 64 atoms are spatially compact after any rigid transform.
+static int build_ligand(lkl_ctx* c, const float* xyz, const float* ele, const float* svdw,
+                        const float* vdwr, const unsigned char* heavy, const unsigned char* flag,
+                        const float* des, const float* asa) {
+    const int nl = c->nl;
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    for (int j = 0; j < nl; ++j)
+        for (int a = 0; a < 3; ++a) lo[a] = std::min(lo[a], xyz[3 * j + a]);
+    std::vector<uint32_t> code(nl);
+    for (int j = 0; j < nl; ++j) {
+        uint32_t q[3];
+        for (int a = 0; a < 3; ++a)
+            q[a] = (uint32_t)std::min(1023.0f, std::max(0.0f, std::floor(xyz[3 * j + a] - lo[a])));
+        code[j] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+    }
+    c->lperm.resize(nl);
+    std::iota(c->lperm.begin(), c->lperm.end(), 0);
+    std::stable_sort(c->lperm.begin(), c->lperm.end(),
+                     [&](int a, int b) { return code[a] < code[b]; });
+    std::vector<float4> lx(nl), lp(nl);
+    std::vector<unsigned char> lf(nl, 0);
+    std::vector<float> ld(nl, 0.f), la(nl, 0.f);
+    for (int s = 0; s < nl; ++s) {
+        const int j = c->lperm[s];
+        lx[s] = make_float4(xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2], ele ? ele[j] : 0.f);
+        lp[s] = make_float4(svdw ? svdw[j] : 0.f, vdwr ? vdwr[j] : 0.f,
+                            (heavy && heavy[j]) ? 1.f : 0.f, 0.f);
+        if (flag) {
+            lf[s] = flag[j];
+            ld[s] = des[j];
+            la[s] = asa[j];
+        }
+    }
+    if (dev_upload(&c->d_lxyzq, lx.data(), (size_t)nl)) return -1;
+    if (dev_upload(&c->d_lpar, lp.data(), (size_t)nl)) return -1;
+    if (flag) {
+        if (dev_upload(&c->d_lflag, lf.data(), (size_t)nl)) return -1;
+        if (dev_upload(&c->d_ldes, ld.data(), (size_t)nl)) return -1;
+        if (dev_upload(&c->d_lasa, la.data(), (size_t)nl)) return -1;
+    }
+    return 0;
+}
+
+static DevComplex dev_view(const lkl_ctx* c) {
+    DevComplex C;
+    memset(&C, 0, sizeof C);
+    C.rxyzq = c->d_rxyzq;
+    C.rpar = c->d_rpar;
+    C.cstart = c->d_cstart;
+    C.rflag = c->d_rflag;
+    C.rdes = c->d_rdes;
+    C.rasa = c->d_rasa;
+    C.nr = c->nr;
+    C.ncx = c->ncx;
+    C.ncy = c->ncy;
+    C.ncz = c->ncz;
+    C.cox = c->cox;
+    C.coy = c->coy;
+    C.coz = c->coz;
+    C.inv_cs = 1.0 / (double)c->cell_size;
+    C.rwin = std::sqrt(WIN2) / c->cell_size;
+    C.rwin2 = C.rwin * C.rwin;
+    C.lxyzq = c->d_lxyzq;
+    C.lpar = c->d_lpar;
+    C.lflag = c->d_lflag;
+    C.ldes = c->d_ldes;
+    C.lasa = c->d_lasa;
+    C.nl = c->nl;
+    C.phi = c->d_phi;
+    C.nx = c->nx;
+    C.ny = c->ny;
+    C.nz = c->nz;
+    C.ox = c->ox;
+    C.oy = c->oy;
+    C.oz = c->oz;
+    C.sp = c->sp;
+    return C;
+}
+
+static void ctx_free(lkl_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    void* bufs[] = {c->d_phi, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
+                    c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
+                    c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static int probe_device() {
+    if (fail_at("probe")) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 0;
+    return strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+static lkl_ctx* ctx_create(const lkl_complex_desc* d) {
+    if (!d) { set_err("lkl_ctx_create: null descriptor"); return nullptr; }
+    if (d->nr < 0 || d->nl <= 0) { set_err("lkl_ctx_create: need nl > 0 and nr >= 0 (nr=%d nl=%d)", d->nr, d->nl); return nullptr; }
+    if (d->nr > 0 && !d->r_xyz) { set_err("lkl_ctx_create: r_xyz is null"); return nullptr; }
+    if (!d->l_xyz) { set_err("lkl_ctx_create: l_xyz is null"); return nullptr; }
+    if (!probe_device()) { set_err("lkl_ctx_create: no gfx950 device available"); return nullptr; }
+    lkl_ctx* c = new lkl_ctx();
+    auto fail = [&]() -> lkl_ctx* { ctx_free(c); return nullptr; };
+    if (hipGetDevice(&c->device) != hipSuccess) { set_err("hipGetDevice failed"); return fail(); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) c->lds_max = prop.sharedMemPerBlock;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_err("hipStreamCreate failed");
+        return fail();
+    }
+    c->nr = d->nr;
+    c->nl = d->nl;
+    if (d->cell_size > 0.0f) c->cell_size = d->cell_size;
+    const size_t nr = (size_t)d->nr;
+    auto cp = [&](std::vector<float>& v, const float* s, size_t n) { if (s) v.assign(s, s + n); };
+    cp(c->r_xyz, d->r_xyz, 3 * nr);
+    cp(c->r_ele, d->r_ele, nr);
+    cp(c->r_svdw, d->r_svdw, nr);
+    cp(c->r_vdwr, d->r_vdwr, nr);
+    if (d->r_heavy) c->r_heavy.assign(d->r_heavy, d->r_heavy + nr);
+    c->has_desolv = d->r_flag && d->r_des && d->r_asa && d->l_flag && d->l_des && d->l_asa;
+    if (c->has_desolv) {
+        c->r_flag.assign(d->r_flag, d->r_flag + nr);
+        cp(c->r_des, d->r_des, nr);
+        cp(c->r_asa, d->r_asa, nr);
+    }
+    c->has_phi = d->phi && d->nx > 1 && d->ny > 1 && d->nz > 1 && d->sp > 0.0;
+    if (c->has_phi) {
+        c->nx = d->nx; c->ny = d->ny; c->nz = d->nz;
+        c->ox = d->ox; c->oy = d->oy; c->oz = d->oz; c->sp = d->sp;
+        if (dev_upload(&c->d_phi, d->phi, (size_t)d->nx * d->ny * d->nz)) return fail();
+    }
+    if (build_receptor(c)) return fail();
+    if (build_ligand(c, d->l_xyz, d->l_ele, d->l_svdw, d->l_vdwr, d->l_heavy,
+                     c->has_desolv ? d->l_flag : nullptr, d->l_des, d->l_asa))
+        return fail();
+    return c;
+}
+
+// Kernel dispatch over the flag word.
+typedef void (*score_fn)(DevComplex, const double*, int, double*, double*, unsigned*, float*,
+                         float*);
+
+template <int DES, bool PA>
+static score_fn pick3(unsigned f) {
+    switch (f & 7u) {
+        case 0: return score_kernel<false, false, false, DES, PA>;
+        case 1: return score_kernel<true, false, false, DES, PA>;
+        case 2: return score_kernel<false, true, false, DES, PA>;
+        case 3: return score_kernel<true, true, false, DES, PA>;
+        case 4: return score_kernel<false, false, true, DES, PA>;
+        case 5: return score_kernel<true, false, true, DES, PA>;
+        case 6: return score_kernel<false, true, true, DES, PA>;
+        default: return score_kernel<true, true, true, DES, PA>;
+    }
+}
+
+static int validate_flags(const lkl_ctx* c, unsigned flags) {
+    if (flags & ~(F_FAR | F_ELEC | F_CLASH | F_DESOLV)) { set_err("unknown flag bits 0x%x", flags); return -1; }
+    if ((flags & F_FAR) && !c->has_phi) { set_err("F_FAR requested but the context has no far field"); return -1; }
+    if ((flags & F_DESOLV) && !c->has_desolv) { set_err("F_DESOLV requested but the context has no desolvation sides"); return -1; }
+    return 0;
+}
+
+// Enqueue one score launch; all pointers are device pointers.
+static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
+                        double* d_outS, hipStream_t st, bool peratom) {
+    if (N <= 0) return 0;
+    if (validate_flags(c, flags)) return -1;
+    const DevComplex C = dev_view(c);
+    int des = DES_NONE;
+    size_t shm = 0;
+    if (flags & F_DESOLV) {
+        if (!d_outS) { set_err("F_DESOLV requires an outS buffer"); return -1; }
+        const size_t need = (size_t)c->nr * sizeof(unsigned);
+        // static LDS of the kernel (reduction slots) needs headroom too
+        if (need + 1024 <= c->lds_max && !getenv("LKL_FORCE_GLOBAL_MIN")) {
+            des = DES_LDS;
+            shm = need;
+        } else {
+            des = DES_GLOBAL;
+            if (dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
+        }
+    }
+    if (peratom && dev_reserve(&c->d_pa_e, &c->cap_pa, (size_t)N * c->nl)) return -1;
+    if (peratom && dev_reserve(&c->d_pa_v, &c->cap_pa_v, (size_t)N * c->nl)) return -1;
+    score_fn fn;
+    if (peratom) fn = pick3<DES_NONE, true>(flags);
+    else if (des == DES_LDS) fn = pick3<DES_LDS, false>(flags);
+    else if (des == DES_GLOBAL) fn = pick3<DES_GLOBAL, false>(flags);
+    else fn = pick3<DES_NONE, false>(flags);
+    if (fail_at("launch")) { set_err("launch_score stage=launch: injected failure"); return -1; }
+    hipLaunchKernelGGL(fn, dim3((unsigned)N), dim3((unsigned)c->block), shm, st, C, d_poses, N,
+                       d_out, d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v);
+    HIPCK("launch", hipGetLastError());
+    return 0;
+}
+
+// Blocking host-buffer score: upload poses, one launch, read back.
+static int score_host(lkl_ctx* c, const double* poses, int N, unsigned flags, double* out,
+                      double* outS, float* pa_e, float* pa_v) {
+    if (N <= 0) { set_err("N must be positive (N=%d)", N); return -1; }
+    if (!poses || !out) { set_err("null poses/out buffer"); return -1; }
+    if ((flags & F_DESOLV) && !outS) { set_err("F_DESOLV requires outS"); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    (void)hipGetLastError();  // drain a stale sticky error (full_score.cu:375)
+    if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
+    if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
+    if ((flags & F_DESOLV) && dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
+    HIPCK("upload", hipMemcpyAsync(c->d_poses, poses, sizeof(double) * 7 * (size_t)N,
+                                   hipMemcpyHostToDevice, c->stream));
+    if (launch_score(c, c->d_poses, N, flags, c->d_out, (flags & F_DESOLV) ? c->d_outS : nullptr,
+                     c->stream, pa_e != nullptr))
+        return -1;
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    HIPCK("readback", hipMemcpy(out, c->d_out, sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToHost));
+    if (flags & F_DESOLV)
+        HIPCK("readback", hipMemcpy(outS, c->d_outS, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost));
+    if (pa_e) {
+        std::vector<float> e((size_t)N * c->nl), v((size_t)N * c->nl);
+        HIPCK("readback", hipMemcpy(e.data(), c->d_pa_e, sizeof(float) * e.size(), hipMemcpyDeviceToHost));
+        HIPCK("readback", hipMemcpy(v.data(), c->d_pa_v, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
+        for (int k = 0; k < N; ++k)
+            for (int s = 0; s < c->nl; ++s) {
+                pa_e[(size_t)k * c->nl + c->lperm[s]] = e[(size_t)k * c->nl + s];
+                pa_v[(size_t)k * c->nl + c->lperm[s]] = v[(size_t)k * c->nl + s];
+            }
+    }
+    return 0;
+}
+
+// ── C ABI: context API ────────────────────────────────────────────────────────
+LKL_API int lkl_device_available(void) { return probe_device(); }
+
+LKL_API const char* lkl_last_error(void) { return t_err.c_str(); }
+
+LKL_API lkl_ctx* lkl_ctx_create(const lkl_complex_desc* desc) { return ctx_create(desc); }
+
+LKL_API void lkl_ctx_destroy(lkl_ctx* ctx) { ctx_free(ctx); }
+
+LKL_API int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, double* out,
+                          double* outS) {
+    if (!ctx) { set_err("lkl_ctx_score: null context"); return -1; }
+    return score_host(ctx, poses, N, flags, out, outS, nullptr, nullptr);
+}
+
+LKL_API int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
+                                double* d_out, double* d_outS, void* stream) {
+    if (!ctx) { set_err("lkl_ctx_score_async: null context"); return -1; }
+    if (N < 0) { set_err("N must be >= 0"); return -1; }
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return launch_score(ctx, d_poses, N, flags, d_out, d_outS, st, false);
+}
+
+LKL_API int lkl_ctx_transform(lkl_ctx* c, const double* poses, int N, double* out_xyz) {
+    if (!c || !poses || !out_xyz || N <= 0) { set_err("lkl_ctx_transform: bad arguments"); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
+    if (dev_reserve(&c->d_xyz, &c->cap_xyz, (size_t)N * c->nl * 3)) return -1;
+    HIPCK("upload", hipMemcpy(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(transform_kernel, dim3((unsigned)N), dim3(256), 0, c->stream, dev_view(c),
+                       c->d_poses, N, c->d_xyz);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    std::vector<double> tmp((size_t)N * c->nl * 3);
+    HIPCK("readback", hipMemcpy(tmp.data(), c->d_xyz, sizeof(double) * tmp.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < N; ++k)
+        for (int s = 0; s < c->nl; ++s)
+            for (int a = 0; a < 3; ++a)
+                out_xyz[((size_t)k * c->nl + c->lperm[s]) * 3 + a] = tmp[((size_t)k * c->nl + s) * 3 + a];
+    return 0;
+}
+
+LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block) {
+    if (!c) { set_err("lkl_ctx_set_tiles: null context"); return -1; }
+    if (!(cell_size > 0.5f && cell_size <= 20.0f)) { set_err("cell_size %.3f out of (0.5, 20]", cell_size); return -1; }
+    if (block != 64 && block != 128 && block != 256 && block != 512) {
+        set_err("block %d not in {64,128,256,512}", block);
+        return -1;
+    }
+    HIPCK("probe", hipSetDevice(c->device));
+    c->block = block;
+    if (cell_size != c->cell_size) {
+        c->cell_size = cell_size;
+        if (build_receptor(c)) return -1;
+    }
+    return 0;
+}
+
+// ── C ABI: stateless wrappers with the reference's signatures ─────────────────
+// Contexts are cached per distinct host-buffer identity (the reference's
+// BatchCache keys, full_score.cu:380-387; the cpydock cache now keys on
+// pointers too, not only (nr, nl) as full_score.cu:594 did).
+static std::mutex g_mu;
+static std::map<std::vector<uint64_t>, lkl_ctx*> g_cache;
+static std::vector<std::vector<uint64_t>> g_lru;
+
+static uint64_t bits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static uint64_t ptr(const void* p) { return (uint64_t)(uintptr_t)p; }
+
+static lkl_ctx* cache_get(const std::vector<uint64_t>& key, const lkl_complex_desc& d) {
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) return it->second;
+    lkl_ctx* c = ctx_create(&d);
+    if (!c) return nullptr;
+    if (g_lru.size() >= 8) {  // bound the number of resident complexes
+        auto old = g_cache.find(g_lru.front());
+        if (old != g_cache.end()) { ctx_free(old->second); g_cache.erase(old); }
+        g_lru.erase(g_lru.begin());
+    }
+    g_cache[key] = c;
+    g_lru.push_back(key);
+    return c;
+}
+
+LKL_API void lkl_cache_clear(void) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto& kv : g_cache) ctx_free(kv.second);
+    g_cache.clear();
+    g_lru.clear();
+}
+
+LKL_API int lkl_batch_score(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                            float sp, const float* r_coords, const float* r_ele,
+                            const float* r_svdw, const float* r_vdwr, const unsigned char* r_heavy,
+                            int nr, const int* cell_start, const int* cell_atoms, int ncx, int ncy,
+                            int ncz, float c_ox, float c_oy, float c_oz, float c_sp,
+                            const float* l_base, const double* poses, const float* l_ele,
+                            const float* l_svdw, const float* l_vdwr,
+                            const unsigned char* l_heavy, int nl, int N, unsigned flags,
+                            double* out) {
+    (void)cell_start; (void)cell_atoms; (void)ncx; (void)ncy; (void)ncz;
+    (void)c_ox; (void)c_oy; (void)c_oz; (void)c_sp;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (N <= 0 || nl <= 0 || nr < 0) { set_err("lkl_batch_score: need N > 0, nl > 0 (N=%d nl=%d nr=%d)", N, nl, nr); return -1; }
+    if (flags & ~(F_FAR | F_ELEC | F_CLASH)) { set_err("lkl_batch_score: unknown flag bits 0x%x", flags); return -1; }
+    lkl_complex_desc d;
+    memset(&d, 0, sizeof d);
+    d.phi = phi; d.nx = nx; d.ny = ny; d.nz = nz; d.ox = ox; d.oy = oy; d.oz = oz; d.sp = sp;
+    d.nr = nr; d.r_xyz = r_coords; d.r_ele = r_ele; d.r_svdw = r_svdw; d.r_vdwr = r_vdwr; d.r_heavy = r_heavy;
+    d.nl = nl; d.l_xyz = l_base; d.l_ele = l_ele; d.l_svdw = l_svdw; d.l_vdwr = l_vdwr; d.l_heavy = l_heavy;
+    std::vector<uint64_t> key = {1, ptr(phi), (uint64_t)nx, (uint64_t)ny, (uint64_t)nz,
+                                 bits(ox), bits(oy), bits(oz), bits(sp), ptr(r_coords), ptr(r_ele),
+                                 ptr(r_svdw), ptr(r_vdwr), ptr(r_heavy), (uint64_t)nr, ptr(l_base),
+                                 ptr(l_ele), ptr(l_svdw), ptr(l_vdwr), ptr(l_heavy), (uint64_t)nl};
+    lkl_ctx* c = cache_get(key, d);
+    if (!c) return -1;
+    return score_host(c, poses, N, flags, out, nullptr, nullptr, nullptr);
+}
+
+LKL_API int lkl_cpydock_solv(const float* r_coords, const unsigned char* r_flag, int nr,
+                             const int* cell_start, const int* cell_atoms, int ncx, int ncy,
+                             int ncz, float c_ox, float c_oy, float c_oz, float c_sp,
+                             const float* l_base, const double* poses,
+                             const unsigned char* l_flag, const float* l_ele, int nl, int N,
+                             const float* r_des, const float* r_asa, const float* l_des,
+                             const float* l_asa, double* outS) {
+    (void)cell_start; (void)cell_atoms; (void)ncx; (void)ncy; (void)ncz;
+    (void)c_ox; (void)c_oy; (void)c_oz; (void)c_sp; (void)l_ele;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (N <= 0 || nl <= 0 || nr <= 0) { set_err("lkl_cpydock_solv: need N, nl, nr > 0 (N=%d nl=%d nr=%d)", N, nl, nr); return -1; }
+    if (!r_flag || !l_flag || !r_des || !r_asa || !l_des || !l_asa || !outS) { set_err("lkl_cpydock_solv: null desolvation array"); return -1; }
+    lkl_complex_desc d;
+    memset(&d, 0, sizeof d);
+    d.nr = nr; d.r_xyz = r_coords; d.nl = nl; d.l_xyz = l_base;
+    d.r_flag = r_flag; d.r_des = r_des; d.r_asa = r_asa;
+    d.l_flag = l_flag; d.l_des = l_des; d.l_asa = l_asa;
+    std::vector<uint64_t> key = {2, ptr(r_coords), ptr(r_flag), (uint64_t)nr, ptr(l_base),
+                                 ptr(l_flag), (uint64_t)nl, ptr(r_des), ptr(r_asa), ptr(l_des), ptr(l_asa)};
+    lkl_ctx* c = cache_get(key, d);
+    if (!c) return -1;
+    std::vector<double> ev((size_t)N * 2);
+    return score_host(c, poses, N, F_DESOLV, ev.data(), outS, nullptr, nullptr);
+}
+
+// Single pre-transformed pose: the identity pose reproduces the f32 input
+// coordinates exactly through the f64 transform.
+static const double k_identity[7] = {1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+LKL_API int lkl_full_score(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                           float sp, const float* r_coords, const float* r_ele,
+                           const float* r_svdw, const float* r_vdwr, const unsigned char* r_heavy,
+                           int nr, const int* cell_start, const int* cell_atoms, int ncx, int ncy,
+                           int ncz, float c_ox, float c_oy, float c_oz, float c_sp,
+                           const float* l_coords, const float* l_ele, const float* l_svdw,
+                           const float* l_vdwr, const unsigned char* l_heavy, int nl,
+                           float* out_elec, float* out_vdw) {
+    (void)cell_start; (void)cell_atoms; (void)ncx; (void)ncy; (void)ncz;
+    (void)c_ox; (void)c_oy; (void)c_oz; (void)c_sp;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (nl <= 0 || nr < 0 || !out_elec || !out_vdw) { set_err("lkl_full_score: bad arguments (nl=%d nr=%d)", nl, nr); return -1; }
+    lkl_complex_desc d;
+    memset(&d, 0, sizeof d);
+    d.phi = phi; d.nx = nx; d.ny = ny; d.nz = nz; d.ox = ox; d.oy = oy; d.oz = oz; d.sp = sp;
+    d.nr = nr; d.r_xyz = r_coords; d.r_ele = r_ele; d.r_svdw = r_svdw; d.r_vdwr = r_vdwr; d.r_heavy = r_heavy;
+    d.nl = nl; d.l_xyz = l_coords; d.l_ele = l_ele; d.l_svdw = l_svdw; d.l_vdwr = l_vdwr; d.l_heavy = l_heavy;
+    // receptor + far field cached on pointer identity; the (pre-transformed)
+    // ligand is replaced on every call, as the reference uploaded it per call
+    // (full_score.cu:169-188)
+    std::vector<uint64_t> key = {3, ptr(phi), (uint64_t)nx, (uint64_t)ny, (uint64_t)nz,
+                                 bits(ox), bits(oy), bits(oz), bits(sp), ptr(r_coords), ptr(r_ele),
+                                 ptr(r_svdw), ptr(r_vdwr), ptr(r_heavy), (uint64_t)nr};
+    lkl_ctx* c = cache_get(key, d);
+    if (!c) return -1;
+    if (!c->has_phi) { set_err("lkl_full_score: a far-field grid is required"); return -1; }
+    c->nl = nl;
+    if (build_ligand(c, l_coords, l_ele, l_svdw, l_vdwr, l_heavy, nullptr, nullptr, nullptr)) return -1;
+    double out2[2];
+    return score_host(c, k_identity, 1, F_FAR | F_ELEC | F_CLASH, out2, nullptr, out_elec, out_vdw);
+}
+
+LKL_API int lkl_far_field(const float* phi, int nx, int ny, int nz, float ox, float oy, float oz,
+                          float sp, const float* coords, const float* charges, int n,
+                          double* result) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (n <= 0 || !coords || !charges || !result) { set_err("lkl_far_field: bad arguments (n=%d)", n); return -1; }
+    lkl_complex_desc d;
+    memset(&d, 0, sizeof d);
+    d.phi = phi; d.nx = nx; d.ny = ny; d.nz = nz; d.ox = ox; d.oy = oy; d.oz = oz; d.sp = sp;
+    d.nr = 0; d.nl = n; d.l_xyz = coords; d.l_ele = charges;
+    std::vector<uint64_t> key = {4, ptr(phi), (uint64_t)nx, (uint64_t)ny, (uint64_t)nz,
+                                 bits(ox), bits(oy), bits(oz), bits(sp)};
+    lkl_ctx* c = cache_get(key, d);
+    if (!c) return -1;
+    if (!c->has_phi) { set_err("lkl_far_field: empty grid"); return -1; }
+    c->nl = n;
+    if (build_ligand(c, coords, charges, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr)) return -1;
+    double out2[2];
+    int rc = score_host(c, k_identity, 1, F_FAR, out2, nullptr, nullptr, nullptr);
+    if (rc == 0) *result = out2[0];
+    return rc;
+}

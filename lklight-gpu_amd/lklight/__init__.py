@@ -1,0 +1,124 @@
+"""lklight — MI355X (gfx950) batched GSO pose scoring for LKlight's all-atom families.
+
+Python view of the C ABI in include/lklight_gpu.h.  Every scoring call runs the
+HIP kernels of liblklight_gpu.so; if the library is missing, or no gfx950
+device is present, calls raise instead of computing anything on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _capi
+from ._capi import (F_CLASH, F_DESOLV, F_ELEC, F_FAR, FLAGS_DNA, FLAGS_PYDOCK, FLAGS_VDW,
+                    GpuError, LibraryMissing, lib, last_error)
+
+FACTOR, EPSILON = 332.0, 4.0
+
+__all__ = ["Context", "device_available", "lib", "last_error", "GpuError", "LibraryMissing",
+           "family_flags", "fold_scores", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
+           "FLAGS_DNA", "FLAGS_PYDOCK", "FLAGS_VDW"]
+
+
+def device_available() -> bool:
+    """lkl_device_available (replaces gpu_score.rs:224-232 cuda_available)."""
+    return bool(lib().lkl_device_available())
+
+
+def family_flags(method: str) -> int:
+    """gpu_family.rs:187-194: dna -> 7, pydock/cpydock -> 3, vdw -> 0, other -> 7."""
+    return {"dna": FLAGS_DNA, "pydock": FLAGS_PYDOCK, "cpydock": FLAGS_PYDOCK,
+            "vdw": FLAGS_VDW}.get(method, FLAGS_DNA)
+
+
+def fold_scores(method: str, e_raw, v, s=None):
+    """Per-pose scores from kernel parts: -(E*332/4 + V) (gpu_score.rs:326-331,
+    gpu_family.rs:283-288); cpydock -(E*332/4 + 0.1 V) + S (cpydock.rs:789-791)."""
+    e = np.asarray(e_raw) * FACTOR / EPSILON
+    if method == "cpydock":
+        return -(e + 0.1 * np.asarray(v)) + np.asarray(s)
+    return -(e + np.asarray(v))
+
+
+class Context:
+    """One uploaded complex (lkl_ctx).  Host arrays are copied at creation."""
+
+    def __init__(self, cx, with_field=True, desolv=False, cell_size=0.0):
+        L = lib()
+        self._keep = []
+        r, l = cx.rec, cx.lig
+
+        def f(a):
+            a = np.ascontiguousarray(a, np.float32)
+            self._keep.append(a)
+            return _capi.ptr(a, C.c_float)
+
+        def u8(a):
+            a = np.ascontiguousarray(a, np.uint8)
+            self._keep.append(a)
+            return _capi.ptr(a, C.c_ubyte)
+
+        d = _capi.ComplexDesc()
+        if with_field:
+            if cx.phi is None:
+                raise ValueError("complex has no far field; call build_field() or set_field()")
+            d.phi = f(cx.phi)
+            d.nx, d.ny, d.nz = cx.dims
+            d.ox, d.oy, d.oz = cx.origin
+            d.sp = 0.5
+        d.nr = r.n
+        d.r_xyz, d.r_ele, d.r_svdw, d.r_vdwr = f(r.xyz), f(r.ele), f(r.seps), f(r.rad)
+        d.r_heavy = u8(r.heavy)
+        d.nl = l.n
+        d.l_xyz, d.l_ele, d.l_svdw, d.l_vdwr = f(l.xyz), f(l.ele), f(l.seps), f(l.rad)
+        d.l_heavy = u8(l.heavy)
+        if desolv:
+            d.r_flag, d.r_des, d.r_asa = u8(r.flag), f(r.des), f(r.asa)
+            d.l_flag, d.l_des, d.l_asa = u8(l.flag), f(l.des), f(l.asa)
+        d.cell_size = float(cell_size)
+        h = L.lkl_ctx_create(C.byref(d))
+        self._keep = []
+        if not h:
+            raise GpuError(f"lkl_ctx_create failed: {last_error()}")
+        self.h = h
+        self.nl, self.nr = l.n, r.n
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lkl_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def score(self, poses, flags):
+        """Blocking: returns (E_raw, V, S) arrays (S None without F_DESOLV)."""
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        n = p.shape[0]
+        out = np.zeros((n, 2), np.float64)
+        s = np.zeros(n, np.float64) if flags & F_DESOLV else None
+        _capi.check(lib().lkl_ctx_score(self.h, _capi.ptr(p, C.c_double), n, flags,
+                                        _capi.ptr(out, C.c_double), _capi.ptr(s, C.c_double)),
+                    "lkl_ctx_score")
+        return out[:, 0].copy(), out[:, 1].copy(), s
+
+    def score_async(self, d_poses, n, flags, d_out, d_outS=None, stream=None):
+        """Device pointers (ints), enqueued on `stream` (hipStream_t as int)."""
+        _capi.check(lib().lkl_ctx_score_async(self.h, C.c_void_p(d_poses), n, flags,
+                                              C.c_void_p(d_out),
+                                              C.c_void_p(d_outS) if d_outS else None,
+                                              C.c_void_p(stream) if stream else None),
+                    "lkl_ctx_score_async")
+
+    def transform(self, poses):
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        out = np.zeros((p.shape[0], self.nl, 3), np.float64)
+        _capi.check(lib().lkl_ctx_transform(self.h, _capi.ptr(p, C.c_double), p.shape[0],
+                                            _capi.ptr(out, C.c_double)), "lkl_ctx_transform")
+        return out
+
+    def set_tiles(self, cell_size, block):
+        _capi.check(lib().lkl_ctx_set_tiles(self.h, float(cell_size), int(block)),
+                    "lkl_ctx_set_tiles")

@@ -1,0 +1,123 @@
+"""ctypes binding of liblklight_gpu.so (include/lklight_gpu.h).
+
+Loading fails loudly: there is no silent substitute for the HIP library.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblklight_gpu.so")
+
+F_FAR, F_ELEC, F_CLASH, F_DESOLV = 1, 2, 4, 16
+FLAGS_DNA, FLAGS_PYDOCK, FLAGS_VDW = 7, 3, 0
+
+_f = C.POINTER(C.c_float)
+_d = C.POINTER(C.c_double)
+_u8 = C.POINTER(C.c_ubyte)
+_i = C.POINTER(C.c_int)
+
+
+class ComplexDesc(C.Structure):
+    """lkl_complex_desc."""
+    _fields_ = [
+        ("phi", _f), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+        ("ox", C.c_double), ("oy", C.c_double), ("oz", C.c_double), ("sp", C.c_double),
+        ("nr", C.c_int), ("r_xyz", _f), ("r_ele", _f), ("r_svdw", _f), ("r_vdwr", _f),
+        ("r_heavy", _u8),
+        ("nl", C.c_int), ("l_xyz", _f), ("l_ele", _f), ("l_svdw", _f), ("l_vdwr", _f),
+        ("l_heavy", _u8),
+        ("r_flag", _u8), ("r_des", _f), ("r_asa", _f),
+        ("l_flag", _u8), ("l_des", _f), ("l_asa", _f),
+        ("cell_size", C.c_float),
+    ]
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises LibraryMissing if it was never built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LibraryMissing(
+            f"{LIB_PATH} is missing: build it with `python lklight-gpu_amd/build.py` "
+            "(there is no CPU substitute for the HIP path)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "lkl_device_available": (C.c_int, []),
+        "lkl_last_error": (C.c_char_p, []),
+        "lkl_cache_clear": (None, []),
+        "lkl_ctx_create": (C.c_void_p, [C.POINTER(ComplexDesc)]),
+        "lkl_ctx_destroy": (None, [C.c_void_p]),
+        "lkl_ctx_score": (C.c_int, [C.c_void_p, _d, C.c_int, C.c_uint, _d, _d]),
+        "lkl_ctx_score_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint,
+                                          C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lkl_ctx_transform": (C.c_int, [C.c_void_p, _d, C.c_int, _d]),
+        "lkl_ctx_set_tiles": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+        "lkl_batch_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int,
+                                      _i, _i, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                      C.c_float, C.c_float, _f, _d, _f, _f, _f, _u8, C.c_int,
+                                      C.c_int, C.c_uint, _d]),
+        "lkl_cpydock_solv": (C.c_int, [_f, _u8, C.c_int, _i, _i, C.c_int, C.c_int, C.c_int,
+                                       C.c_float, C.c_float, C.c_float, C.c_float, _f, _d, _u8,
+                                       _f, C.c_int, C.c_int, _f, _f, _f, _f, _d]),
+        "lkl_full_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int, _i,
+                                     _i, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                     C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int, _f,
+                                     _f]),
+        "lkl_far_field": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                    C.c_float, C.c_float, _f, _f, C.c_int, _d]),
+        "lkl_field_dims": (C.c_longlong, [_f, C.c_int, _d, _i]),
+        "lkl_field_build": (C.c_int, [_f, _f, C.c_int, _f]),
+        "lkl_synth_rsa": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_uint64, C.c_int64, _d, C.POINTER(C.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def ptr(a, ctype):
+    """Pointer to a contiguous numpy array (None passes NULL)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def last_error() -> str:
+    return lib().lkl_last_error().decode(errors="replace")
+
+
+class GpuError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str):
+    if rc != 0:
+        raise GpuError(f"{what} failed: {last_error()}")
+
+
+def as_f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def as_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def as_u8(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)

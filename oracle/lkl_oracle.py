@@ -1,0 +1,205 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liblkl_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker of the HIP path.  Imported by tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the product.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liblkl_oracle.so")
+
+DNA, VDW, PYDOCK, CPYDOCK = 0, 1, 2, 3
+FAMILY = {"dna": DNA, "vdw": VDW, "pydock": PYDOCK, "cpydock": CPYDOCK}
+
+_d = C.POINTER(C.c_double)
+
+
+class Mol(C.Structure):
+    _fields_ = [("n", C.c_int), ("xyz", _d), ("ele", _d), ("eps", _d), ("seps", _d),
+                ("rad", _d), ("heavy", C.POINTER(C.c_ubyte)), ("hyd", C.POINTER(C.c_int)),
+                ("des", _d), ("asa", _d)]
+
+
+class Field(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("n", C.c_int64 * 3), ("spacing", C.c_double),
+                ("phi", C.POINTER(C.c_float))]
+
+
+class Parts(C.Structure):
+    _fields_ = [("e_raw", C.c_double), ("v", C.c_double), ("s", C.c_double),
+                ("n_near", C.c_int64), ("n_far", C.c_int64), ("n_cand", C.c_int64)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_field_dims.restype = C.c_int64
+        L.orc_field_dims.argtypes = [_d, C.c_int, _d, C.POINTER(C.c_int64)]
+        L.orc_field_fill.restype = None
+        L.orc_field_fill.argtypes = [_d, _d, C.c_int, _d, C.POINTER(C.c_int64),
+                                     C.POINTER(C.c_float)]
+        L.orc_field_sample.restype = C.c_double
+        L.orc_field_sample.argtypes = [C.POINTER(Field), C.c_double, C.c_double, C.c_double]
+        L.orc_crec_build.restype = C.c_int
+        L.orc_crec_build.argtypes = [_d, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_transform.restype = None
+        L.orc_transform.argtypes = [_d, C.c_int, _d, _d]
+        for name in ("orc_dna_energy_grid", "orc_pydock_energy_grid", "orc_cpydock_energy_grid"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [C.POINTER(Mol), C.POINTER(Mol), C.POINTER(Field), _d, C.POINTER(Parts)]
+        L.orc_vdw_energy_grid.restype = None
+        L.orc_vdw_energy_grid.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d, C.POINTER(Parts)]
+        L.orc_fold_score.restype = C.c_double
+        L.orc_fold_score.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double]
+        L.orc_batch_energy_grid.restype = None
+        L.orc_batch_energy_grid.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol),
+                                            C.POINTER(Field), _d, C.c_int, _d, _d, _d,
+                                            C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int]
+        L.orc_bruteforce_parts.restype = None
+        L.orc_bruteforce_parts.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol),
+                                           C.POINTER(Field), _d, C.POINTER(Parts)]
+        L.orc_dna_energy_exact.restype = C.c_double
+        L.orc_dna_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
+        L.orc_nearcells_pair_count.restype = C.c_int64
+        L.orc_nearcells_pair_count.argtypes = [_d, C.c_int, C.c_double]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class OMol:
+    """Oracle view of a molecule (keeps the f64 arrays alive)."""
+
+    def __init__(self, m):
+        self.arrs = dict(
+            xyz=np.ascontiguousarray(m.xyz, np.float64), ele=np.ascontiguousarray(m.ele, np.float64),
+            eps=np.ascontiguousarray(m.eps, np.float64), seps=np.ascontiguousarray(m.seps, np.float64),
+            rad=np.ascontiguousarray(m.rad, np.float64), heavy=np.ascontiguousarray(m.heavy, np.uint8),
+            hyd=np.ascontiguousarray(m.hyd, np.int32), des=np.ascontiguousarray(m.des, np.float64),
+            asa=np.ascontiguousarray(m.asa, np.float64))
+        a = self.arrs
+        self.s = Mol(m.xyz.shape[0], _p(a["xyz"]), _p(a["ele"]), _p(a["eps"]), _p(a["seps"]),
+                     _p(a["rad"]), _p(a["heavy"], C.c_ubyte), _p(a["hyd"], C.c_int),
+                     _p(a["des"]), _p(a["asa"]))
+
+
+def field_build(xyz, q):
+    """grid_dna.rs:44-149 -> (phi f32 flat, origin, dims)."""
+    L = lib()
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    q = np.ascontiguousarray(q, np.float64)
+    o = np.zeros(3, np.float64)
+    d = np.zeros(3, np.int64)
+    n = L.orc_field_dims(_p(xyz), xyz.shape[0], _p(o), _p(d, C.c_int64))
+    phi = np.empty(int(n), np.float32)
+    L.orc_field_fill(_p(xyz), _p(q), xyz.shape[0], _p(o), _p(d, C.c_int64), _p(phi, C.c_float))
+    return phi, tuple(o), tuple(int(v) for v in d)
+
+
+class OField:
+    def __init__(self, phi, origin, dims):
+        self.phi = np.ascontiguousarray(phi, np.float32)
+        self.s = Field((C.c_double * 3)(*origin), (C.c_int64 * 3)(*dims), 0.5,
+                       _p(self.phi, C.c_float))
+
+    def sample(self, x, y, z):
+        return lib().orc_field_sample(C.byref(self.s), x, y, z)
+
+
+class Oracle:
+    """CPU grid path of one complex."""
+
+    def __init__(self, cx):
+        self.rec, self.lig = OMol(cx.rec), OMol(cx.lig)
+        self.field = OField(cx.phi, cx.origin, cx.dims) if cx.phi is not None else None
+
+    def _fp(self):
+        return C.byref(self.field.s) if self.field is not None else None
+
+    def parts(self, method, pose):
+        """One pose of `method`'s energy_grid -> Parts (e_raw, v, s, n_near, n_far)."""
+        L = lib()
+        p = np.ascontiguousarray(pose, np.float64)
+        out = Parts()
+        if method == "vdw":
+            L.orc_vdw_energy_grid(C.byref(self.rec.s), C.byref(self.lig.s), _p(p), C.byref(out))
+        else:
+            fn = {"dna": L.orc_dna_energy_grid, "pydock": L.orc_pydock_energy_grid,
+                  "cpydock": L.orc_cpydock_energy_grid}[method]
+            fn(C.byref(self.rec.s), C.byref(self.lig.s), self._fp(), _p(p), C.byref(out))
+        return out
+
+    def batch(self, method, poses, nthreads=1, counts=False):
+        """Batched energy_grid -> dict of arrays e, v, s (+ n_near, n_far)."""
+        L = lib()
+        p = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
+        n = p.shape[0]
+        e, v, s = np.zeros(n), np.zeros(n), np.zeros(n)
+        nn = np.zeros(n, np.int64)
+        nf = np.zeros(n, np.int64)
+        L.orc_batch_energy_grid(FAMILY[method], C.byref(self.rec.s), C.byref(self.lig.s),
+                                self._fp() if method != "vdw" else None, _p(p), n, _p(e), _p(v),
+                                _p(s), _p(nn, C.c_int64) if counts else None,
+                                _p(nf, C.c_int64) if counts else None, int(nthreads))
+        out = {"e": e, "v": v, "s": s}
+        if counts:
+            out["n_near"], out["n_far"] = nn, nf
+        return out
+
+    def bruteforce(self, method, pose):
+        out = Parts()
+        p = np.ascontiguousarray(pose, np.float64)
+        lib().orc_bruteforce_parts(FAMILY[method], C.byref(self.rec.s), C.byref(self.lig.s),
+                                   self._fp() if method != "vdw" else None, _p(p), C.byref(out))
+        return out
+
+    def dna_exact(self, pose):
+        p = np.ascontiguousarray(pose, np.float64)
+        return lib().orc_dna_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
+
+    def transform(self, pose):
+        p = np.ascontiguousarray(pose, np.float64)
+        xyz = self.lig.arrs["xyz"]
+        out = np.zeros_like(xyz)
+        lib().orc_transform(_p(xyz), xyz.shape[0], _p(p), _p(out))
+        return out
+
+
+def fold(method, e, v, s=0.0):
+    return lib().orc_fold_score(FAMILY[method], e, v, s)
+
+
+def nearcells_pair_count(xyz, cell):
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    return lib().orc_nearcells_pair_count(_p(xyz), xyz.shape[0], float(cell))
+
+
+def crec_build(xyz):
+    L = lib()
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    nc = (C.c_int * 3)()
+    co = (C.c_float * 3)()
+    ln = L.orc_crec_build(_p(xyz), xyz.shape[0], nc, co, None, None)
+    cs = np.zeros(ln, np.int32)
+    ca = np.zeros(max(1, xyz.shape[0]), np.int32)
+    L.orc_crec_build(_p(xyz), xyz.shape[0], nc, co, _p(cs, C.c_int), _p(ca, C.c_int))
+    return cs, ca[:xyz.shape[0]], tuple(nc), tuple(co)

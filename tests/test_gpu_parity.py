@@ -1,0 +1,306 @@
+"""Parity of the gfx950 HIP path with the CPU oracle (the reference CPU grid path).
+
+Bar (stated per test): per-pose score within 1e-5 relative of the oracle's f64
+CPU grid value (absolute floor 1 kcal/mol for scores near 0), each component
+within the same budget, integer/bit outputs (transform, phi, invariances)
+bit-exact.  All calls go through the C ABI of liblklight_gpu.so.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import lklight
+from lklight import _capi, synth
+from oracle import lkl_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+FLOOR = 1.0
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_scores(method, gpu, orc, rel=REL, floor=FLOOR):
+    """gpu/orc: dicts e, v, s -> folded scores + component budgets."""
+    g = lklight.fold_scores(method, gpu["e"], gpu["v"], gpu.get("s"))
+    o = lklight.fold_scores(method, orc["e"], orc["v"], orc.get("s"))
+    tol = rel * np.maximum(np.abs(o), floor)
+    err = np.abs(g - o)
+    say(method, "max rel err %.3e" % float((err / np.maximum(np.abs(o), floor)).max()))
+    assert (err <= tol).all(), (method, err.max(), np.argmax(err / tol))
+    assert (np.abs(gpu["e"] - orc["e"]) * 83.0 <= tol).all()
+    assert (np.abs(gpu["v"] - orc["v"]) <= tol).all()
+    if gpu.get("s") is not None:
+        assert (np.abs(gpu["s"] - orc["s"]) <= tol).all()
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    say("build complex 'small' + oracle field")
+    cx = synth.make_config("small")
+    phi, o, d = O.field_build(cx.rec.xyz, cx.rec.ele)
+    cx.set_field(phi, o, d)
+    poses = synth.make_poses(64, cx.meta["rec_axes"], 5)
+    return cx, O.Oracle(cx), poses
+
+
+def run(ctx, poses, flags):
+    e, v, s = ctx.score(poses, flags)
+    return {"e": e, "v": v, "s": s}
+
+
+def test_transform_bitwise(small):
+    cx, orc, poses = small
+    say("transform: ctx create")
+    ctx = lklight.Context(cx, with_field=False)
+    rng = np.random.default_rng(2)
+    extra = np.concatenate([synth.random_quaternions(8, rng), rng.normal(0, 30, (8, 3))], 1)
+    P = np.concatenate([poses[:24], extra])
+    got = ctx.transform(P)
+    for k in range(P.shape[0]):
+        assert np.array_equal(got[k], orc.transform(P[k])), k
+    ctx.close()
+
+
+@pytest.mark.parametrize("name", ["tiny", "small", "c1"])
+def test_field_build_bitwise(gpu, name):
+    cx = synth.make_config(name)
+    say("field build", name)
+    phi_o, o, d = O.field_build(cx.rec.xyz, cx.rec.ele)
+    phi_g = cx.build_field()
+    assert cx.origin == o and cx.dims == d
+    assert np.array_equal(phi_g.view(np.uint32), phi_o.view(np.uint32))
+
+
+@pytest.mark.parametrize("method", ["dna", "vdw", "pydock"])
+def test_family_parity(small, method):
+    cx, orc, poses = small
+    say("family parity", method)
+    ctx = lklight.Context(cx)
+    gpu = run(ctx, poses, lklight.family_flags(method))
+    ref = orc.batch(method, poses, nthreads=4)
+    if method == "vdw":
+        assert (gpu["e"] == 0).all()
+    assert_scores(method, gpu, ref)
+    ctx.close()
+
+
+def test_cpydock_parity(small):
+    cx, orc, poses = small
+    say("cpydock parity (fused E, V, S)")
+    ctx = lklight.Context(cx, desolv=True)
+    gpu = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
+    ref = orc.batch("cpydock", poses, nthreads=4)
+    assert_scores("cpydock", gpu, ref)
+    # desolvation-only launch gives the same S bit for bit
+    s_only = run(ctx, poses, lklight.F_DESOLV)["s"]
+    assert np.array_equal(s_only, gpu["s"])
+    # the global-scratch variant of the receptor minima gives the same bits
+    os.environ["LKL_FORCE_GLOBAL_MIN"] = "1"
+    try:
+        say("cpydock global-min path")
+        glob_ = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
+    finally:
+        del os.environ["LKL_FORCE_GLOBAL_MIN"]
+    for k in ("e", "v", "s"):
+        assert np.array_equal(glob_[k], gpu[k]), k
+    ctx.close()
+
+
+def test_tile_and_order_invariance(small):
+    """Fixed-point per-pose sums: bitwise identical for every cell edge, block
+    size, and input atom order."""
+    cx, orc, poses = small
+    say("tile invariance")
+    ctx = lklight.Context(cx, desolv=True)
+    flags = lklight.FLAGS_DNA
+    base = run(ctx, poses, flags)
+    base_c = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
+    for cell, block in [(10.0, 256), (3.4, 64), (6.0, 512), (4.0, 128)]:
+        ctx.set_tiles(cell, block)
+        r = run(ctx, poses, flags)
+        rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
+        for k in ("e", "v"):
+            assert np.array_equal(r[k], base[k]), (cell, block, k)
+        for k in ("e", "v", "s"):
+            assert np.array_equal(rc[k], base_c[k]), (cell, block, k)
+    ctx.close()
+    say("order invariance")
+    rng = np.random.default_rng(0)
+    pr, pl = rng.permutation(cx.rec.n), rng.permutation(cx.lig.n)
+
+    def perm(m, p):
+        return synth.Mol(xyz=m.xyz[p], ele=m.ele[p], rad=m.rad[p], seps=m.seps[p],
+                         heavy=m.heavy[p], hyd=m.hyd, des=m.des[p], asa=m.asa[p])
+    cx2 = synth.Complex(rec=perm(cx.rec, pr), lig=perm(cx.lig, pl))
+    cx2.set_field(cx.phi, cx.origin, cx.dims)
+    ctx2 = lklight.Context(cx2)
+    r2 = run(ctx2, poses, flags)
+    assert np.array_equal(r2["e"], base["e"]) and np.array_equal(r2["v"], base["v"])
+    ctx2.close()
+
+
+def test_determinism_and_batch_split(small):
+    cx, orc, poses = small
+    say("determinism")
+    ctx = lklight.Context(cx)
+    a = run(ctx, poses, lklight.FLAGS_DNA)
+    b = run(ctx, poses, lklight.FLAGS_DNA)
+    c = run(ctx, poses[:17], lklight.FLAGS_DNA)
+    d = run(ctx, poses[17:], lklight.FLAGS_DNA)
+    for k in ("e", "v"):
+        assert np.array_equal(a[k], b[k])
+        assert np.array_equal(a[k], np.concatenate([c[k], d[k]]))
+    ctx.close()
+
+
+def _f(a):
+    return _capi.ptr(np.ascontiguousarray(a, np.float32), C.c_float)
+
+
+def test_stateless_wrappers(small):
+    """lkl_batch_score / lkl_cpydock_solv / lkl_full_score / lkl_far_field with the
+    reference argument lists (full_score.cu:359-371, 571-581, 140-150,
+    far_field.cu:59-63)."""
+    cx, orc, poses = small
+    L = lklight.lib()
+    keep = {}
+
+    def f(name, a, dt=np.float32):
+        keep[name] = np.ascontiguousarray(a, dt)
+        return keep[name]
+
+    phi = f("phi", cx.phi)
+    rx, re_, rs, rr = f("rx", cx.rec.xyz), f("re", cx.rec.ele), f("rs", cx.rec.seps), f("rr", cx.rec.rad)
+    rh = f("rh", cx.rec.heavy, np.uint8)
+    lx, le, ls, lr = f("lx", cx.lig.xyz), f("le", cx.lig.ele), f("ls", cx.lig.seps), f("lr", cx.lig.rad)
+    lh = f("lh", cx.lig.heavy, np.uint8)
+    cs, ca, nc, co = O.crec_build(cx.rec.xyz)
+    P = np.ascontiguousarray(poses, np.float64)
+    fp = lambda a: _capi.ptr(a, C.c_float)  # noqa: E731
+    up = lambda a: _capi.ptr(a, C.c_ubyte)  # noqa: E731
+    ip = lambda a: _capi.ptr(a, C.c_int)  # noqa: E731
+    dp = lambda a: _capi.ptr(a, C.c_double)  # noqa: E731
+    ctx = lklight.Context(cx, desolv=True)
+    for method in ("dna", "pydock", "vdw"):
+        say("lkl_batch_score", method)
+        flags = lklight.family_flags(method)
+        out = np.zeros((P.shape[0], 2))
+        rc = L.lkl_batch_score(fp(phi), *cx.dims, *[float(v) for v in cx.origin], 0.5,
+                               fp(rx), fp(re_), fp(rs), fp(rr), up(rh), cx.rec.n, ip(cs), ip(ca),
+                               *nc, *co, 10.0, fp(lx), dp(P), fp(le), fp(ls), fp(lr), up(lh),
+                               cx.lig.n, P.shape[0], flags, dp(out))
+        assert rc == 0, lklight.last_error()
+        ref = run(ctx, P, flags)
+        assert np.array_equal(out[:, 0], ref["e"]) and np.array_equal(out[:, 1], ref["v"])
+    say("lkl_cpydock_solv")
+    rf, rd, ra = f("rf", cx.rec.flag, np.uint8), f("rd", cx.rec.des), f("ra", cx.rec.asa)
+    lf, ld, la = f("lf", cx.lig.flag, np.uint8), f("ld", cx.lig.des), f("la", cx.lig.asa)
+    outS = np.zeros(P.shape[0])
+    rc = L.lkl_cpydock_solv(fp(rx), up(rf), cx.rec.n, ip(cs), ip(ca), *nc, *co, 10.0, fp(lx), dp(P),
+                            up(lf), fp(le), cx.lig.n, P.shape[0], fp(rd), fp(ra), fp(ld), fp(la),
+                            dp(outS))
+    assert rc == 0, lklight.last_error()
+    assert np.array_equal(outS, run(ctx, P, lklight.F_DESOLV)["s"])
+    ctx.close()
+    say("lkl_full_score / lkl_far_field")
+    for k in range(3):
+        tl = np.float32(orc.transform(P[k]))  # pre-transformed f32 coordinates
+        oe, ov = np.zeros(cx.lig.n, np.float32), np.zeros(cx.lig.n, np.float32)
+        rc = L.lkl_full_score(fp(phi), *cx.dims, *[float(v) for v in cx.origin], 0.5, fp(rx),
+                              fp(re_), fp(rs), fp(rr), up(rh), cx.rec.n, ip(cs), ip(ca), *nc, *co,
+                              10.0, fp(tl), fp(le), fp(ls), fp(lr), up(lh), cx.lig.n, fp(oe), fp(ov))
+        assert rc == 0, lklight.last_error()
+        lig_t = synth.Mol(xyz=tl.astype(np.float64), ele=cx.lig.ele, rad=cx.lig.rad,
+                          seps=cx.lig.seps, heavy=cx.lig.heavy, hyd=cx.lig.hyd, des=cx.lig.des,
+                          asa=cx.lig.asa)
+        cxt = synth.Complex(rec=cx.rec, lig=lig_t)
+        cxt.set_field(cx.phi, cx.origin, cx.dims)
+        ot = O.Oracle(cxt)
+        ref = ot.parts("dna", [1, 0, 0, 0, 0, 0, 0])
+        sc_g = -(oe.astype(np.float64).sum() * 83.0 + ov.astype(np.float64).sum())
+        sc_o = O.fold("dna", ref.e_raw, ref.v)
+        assert abs(sc_g - sc_o) <= 1e-5 * max(abs(sc_o), FLOOR)
+        res = C.c_double(0.0)
+        rc = L.lkl_far_field(fp(phi), *cx.dims, *[float(v) for v in cx.origin], 0.5, fp(tl),
+                             fp(le), cx.lig.n, C.byref(res))
+        assert rc == 0, lklight.last_error()
+        far = sum(cx.lig.ele[j] * ot.field.sample(*tl[j].astype(np.float64))
+                  for j in range(cx.lig.n) if cx.lig.ele[j] != 0)
+        # per-atom fixed-point quantisation: <= 2^-38 per atom
+        assert abs(res.value - far) <= 1e-9 * max(1.0, abs(far))
+    L.lkl_cache_clear()  # the cache keys on host pointers that die with this test
+
+
+def test_edge_cases(small):
+    cx, orc, poses = small
+    say("edge cases")
+    ctx = lklight.Context(cx, desolv=True)
+    # a single pose; a pose far from everything (no near pairs, far field off-grid)
+    far = np.array([[1, 0, 0, 0, 500.0, -400.0, 300.0]])
+    r = run(ctx, far, lklight.FLAGS_DNA)
+    assert r["e"][0] == 0.0 and r["v"][0] == 0.0
+    one = run(ctx, poses[:1], lklight.FLAGS_DNA)
+    ref = orc.batch("dna", poses[:1])
+    assert_scores("dna", one, ref)
+    # ligand straddling the far-field box edge: 40 A outside the receptor box
+    edge = poses[:8].copy()
+    edge[:, 4] += 40.0
+    assert_scores("pydock", run(ctx, edge, lklight.FLAGS_PYDOCK), orc.batch("pydock", edge))
+    # errors: N = 0 through the blocking call, unknown flag bits, F_FAR without a field
+    with pytest.raises(lklight.GpuError):
+        ctx.score(np.zeros((0, 7)), lklight.FLAGS_DNA)
+    with pytest.raises(lklight.GpuError, match="unknown flag"):
+        ctx.score(poses[:2], 0x100)
+    ctx.close()
+    nofield = lklight.Context(cx, with_field=False)
+    with pytest.raises(lklight.GpuError, match="no far field"):
+        nofield.score(poses[:2], lklight.FLAGS_DNA)
+    with pytest.raises(lklight.GpuError, match="no desolvation"):
+        nofield.score(poses[:2], lklight.F_DESOLV)
+    nofield.close()
+
+
+def test_config1_parity(gpu):
+    """configs[0] complex: 1,500-atom receptor, 800-atom ligand, dna, 50 poses."""
+    cx = synth.make_config("c1")
+    say("c1 oracle field")
+    phi, o, d = O.field_build(cx.rec.xyz, cx.rec.ele)
+    cx.set_field(phi, o, d)
+    poses = synth.make_poses(50, cx.meta["rec_axes"], 42)
+    orc = O.Oracle(cx)
+    ctx = lklight.Context(cx, desolv=True)
+    for method in ("dna", "vdw", "pydock", "cpydock"):
+        say("c1", method)
+        flags = lklight.family_flags(method) | (lklight.F_DESOLV if method == "cpydock" else 0)
+        assert_scores(method, run(ctx, poses, flags), orc.batch(method, poses, nthreads=8))
+    ctx.close()
+
+
+def test_config2_full_size(gpu):
+    """12,625-atom receptor / 1,500-atom ligand at full size: oracle on a pose
+    subset (phi built on the GPU, bit-equal to the oracle's by
+    test_field_build_bitwise), size-independent properties on the full batch."""
+    cx = synth.make_config("c2")
+    say("c2 field build (GPU)")
+    cx.build_field()
+    poses = synth.make_poses(1200, cx.meta["rec_axes"], 7)
+    ctx = lklight.Context(cx)
+    say("c2 score 1200 poses")
+    full = run(ctx, poses, lklight.FLAGS_DNA)
+    orc = O.Oracle(cx)
+    sub = poses[::40]
+    say("c2 oracle subset", sub.shape[0])
+    assert_scores("dna", {"e": full["e"][::40], "v": full["v"][::40]},
+                  orc.batch("dna", sub, nthreads=8))
+    # batch split and repeat invariance at full size
+    again = run(ctx, poses[600:], lklight.FLAGS_DNA)
+    assert np.array_equal(again["e"], full["e"][600:]) and np.array_equal(again["v"], full["v"][600:])
+    ctx.close()
