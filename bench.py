@@ -65,7 +65,21 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=-1)
+    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="run only a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        import torch
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": src.numel() * 4, "copy_bytes_written": dst.numel() * 4,
+                          "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus %d is not supported "
                           "(poses shard across GPUs with no exchange, but no multi-GPU path is "
@@ -89,22 +103,37 @@ def main():
     log("phi %s built on GPU in %.2f s" % (cx.dims, time.perf_counter() - t))
     desolv = args.family == "cpydock"
     ctx = lklight.Context(cx, desolv=desolv)
-    if args.cell or args.block:
-        ctx.set_tiles(args.cell or 5.0, args.block or 256)
     flags = lklight.family_flags(args.family) | (lklight.F_DESOLV if desolv else 0)
 
     K, W, N = args.steps, args.warmup, POSES_PER_STEP
     poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
+    tune = None
+    if args.cell or args.block or args.chunk >= 0:
+        tiles = (args.cell or 4.0, args.block or 256, max(args.chunk, 0))
+        ctx.set_tiles(*tiles)
+    elif not args.no_tune:
+        t = time.perf_counter()
+        cell, block, chunk, table = ctx.autotune(poses[:N], flags)
+        tiles = (cell, block, chunk)
+        tune = {"table": [[round(a, 2), int(b), int(c), round(d, 4)] for a, b, c, d in table],
+                "seconds": round(time.perf_counter() - t, 2)}
+        log("autotuner: cell %.1f A, block %d, chunk %d (%.1f s)" % (cell, block, chunk, tune["seconds"]))
+    else:
+        tiles = (5.0, 256, 0)
     d_poses = torch.from_numpy(poses).to(dev)
     d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
     d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream()
+    # a dedicated (default-priority) stream: its handle is non-null, so the
+    # kernels and the timing events share it
+    stream = torch.cuda.Stream()
     sh = stream.cuda_stream
+    assert sh, "expected a non-null stream handle"
 
     def step(i):
         ctx.score_async(d_poses[i * N:(i + 1) * N].data_ptr(), N, flags, d_out[i].data_ptr(),
                         d_s[i].data_ptr() if desolv else None, sh)
 
+    torch.cuda.synchronize()
     for i in range(W):
         step(i)
     torch.cuda.synchronize()
@@ -159,7 +188,9 @@ def main():
         "config": {"workload": "c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "
                                "(6 swarms x 200 glowworms), phi %s at 0.5 A"
                                % (args.family, N, "x".join(map(str, cx.dims))),
-                   "family": args.family, "flags": flags, "poses_per_step": N},
+                   "family": args.family, "flags": flags, "poses_per_step": N,
+                   "cell_edge_A": tiles[0], "block": tiles[1], "ligand_atoms_per_tile": tiles[2],
+                   "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_pose": b_pose,

@@ -150,9 +150,84 @@ int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned fla
  * order of the descriptor), computed by the device transform. */
 int lkl_ctx_transform(lkl_ctx* ctx, const double* poses, int N, double* out_xyz);
 
-/* Set the receptor cell edge and workgroup size used by later score calls
- * (autotuner hook).  cell_size in A (> 0), block in {64,128,256,512}. */
-int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block);
+/* Set the receptor cell edge (A, in (0.5, 20]), workgroup size (64, 128, 256
+ * or 512) and ligand atoms per workgroup tile (0 = the whole ligand) used by
+ * later score calls.  Scores are bitwise identical for every setting. */
+int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block, int chunk);
+
+/* Autotuner: time every (receptor cell edge, workgroup size, ligand tile)
+ * candidate on N host poses with `flags`, keep the fastest in the context and
+ * report it.  Scores are bitwise identical for every candidate.  table (may be
+ * NULL) receives up to 128 rows of (cell A, block, chunk, median ms); *n_table
+ * their count.  No reference counterpart (full_score.cu:442 hard-codes one
+ * 256-thread shape). */
+int lkl_ctx_autotune(lkl_ctx* ctx, const double* poses, int N, unsigned flags,
+                     float* best_cell, int* best_block, int* best_chunk, float* table,
+                     int* n_table);
+
+/* ── receptor far field and synthetic inputs ──────────────────────────────── */
+
+/* grid_dna.rs:44-61: origin (3 doubles) and node counts (3 ints) of the 0.5 A
+ * field box around the receptor; returns the node count or -1. */
+long long lkl_field_dims(const float* r_xyz, int nr, double* origin3, int* dims3);
+
+/* Replaces ReceptorField::build (grid_dna.rs:44-149), run on the GPU: phi_out
+ * (lkl_field_dims() floats, x fastest) receives, per node, the f32 sum in
+ * atom order of q_i/d^2 over 10 < d <= 30 A -- the reference's values bit for
+ * bit.  Returns 0 / -1. */
+int lkl_field_build(const float* r_xyz, const float* r_ele, int nr, float* phi_out);
+
+/* Seeded random sequential addition of n atoms in the ellipsoid with semi-axes
+ * (a, b, c) at minimum separation dmin, coordinates quantised to 2^-10 A
+ * (synthetic benchmark inputs; no reference counterpart).  Returns the number
+ * of atoms placed; *attempts receives the candidates drawn. */
+int lkl_synth_rsa(int n, double a, double b, double c, double dmin, unsigned long long seed,
+                  long long max_attempts, double* xyz, long long* attempts);
+
+/* ── GSO driver (lib.rs / swarm.rs / glowworm.rs) ──────────────────────────── */
+
+typedef struct lkl_gso lkl_gso;
+
+/* Caller's CPU scoring path (the reference's per-pose energy_grid fallback,
+ * dna.rs:1081-1085): poses n x 7 (w, x, y, z, tx, ty, tz) -> scores n.
+ * Returns 0 on success. */
+typedef int (*lkl_score_fallback)(void* user, const double* poses, int n, double* scores);
+
+/* GSO over n_swarms independent swarms of one complex (GSO::new, lib.rs:47-64;
+ * every swarm seeds its own StdRng with `seed`, simulator.rs:184).  positions:
+ * sum(swarm_sizes) rows (tx, ty, tz, w, x, y, z) as in swarm.rs:38-42.  family:
+ * 0 dna, 1 vdw, 2 pydock, 3 cpydock (flag word of gpu_family.rs:187-194 and the
+ * score fold).  ctx scores on the GPU; fallback (may be NULL) is called only if
+ * the GPU call fails or ctx is NULL.  Returns NULL on bad arguments. */
+lkl_gso* lkl_gso_create(lkl_ctx* ctx, int family, int n_swarms, const int* swarm_sizes,
+                        const double* positions, unsigned long long seed,
+                        lkl_score_fallback fallback, void* fallback_user);
+void lkl_gso_destroy(lkl_gso* gso);
+/* Swarm::update_luciferin (swarm.rs:70-103) for all swarms: the glowworms that
+ * moved (or all, at step 0) are scored in one launch.  0 / -1. */
+int lkl_gso_update_luciferin(lkl_gso* gso);
+/* Swarm::movement_phase (swarm.rs:105-174) for every swarm.  0 / -1. */
+int lkl_gso_movement_phase(lkl_gso* gso);
+/* GSO::run (lib.rs:66-78): `steps` x (update_luciferin, movement_phase). */
+int lkl_gso_run(lkl_gso* gso, int steps);
+/* Number of glowworms over all swarms. */
+int lkl_gso_size(const lkl_gso* gso);
+/* Per glowworm (swarm order): pose (w, x, y, z, tx, ty, tz), luciferin, last
+ * score, vision range, neighbour count, moved flag; any pointer may be NULL. */
+int lkl_gso_state(const lkl_gso* gso, double* pose7, double* luciferin, double* scoring,
+                  double* vision, int* n_neighbors, unsigned char* moved);
+/* Replace the poses of the glowworms selected by mask (NULL = all) and mark
+ * them moved (no reference counterpart: used by surrogate-screened moves). */
+int lkl_gso_set_poses(lkl_gso* gso, const double* pose7, const unsigned char* mask);
+/* GPU launches, fallback calls and poses scored on the GPU so far. */
+int lkl_gso_stats(const lkl_gso* gso, long long* gpu_launches, long long* fallback_calls,
+                  long long* scored);
+/* Swarm::save (swarm.rs:232-271): <directory>/gso_<step>.out of one swarm. */
+int lkl_gso_save(const lkl_gso* gso, int swarm, int step, const char* directory);
+/* rand 0.7.3 StdRng gen::<f64>() stream of seed_from_u64(seed) (qt.rs:474-487 KAT). */
+void lkl_rng_f64s(unsigned long long seed, int n, double* out);
+/* Quaternion::slerp (qt.rs:80-104) of (w, x, y, z) quaternions. */
+void lkl_quat_slerp(const double* a, const double* b, double t, double* out);
 
 #ifdef __cplusplus
 }

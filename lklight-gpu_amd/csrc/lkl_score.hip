@@ -104,8 +104,11 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
             const int row = (iz * C.ncy + iy) * C.ncx;
             int p = C.cstart[row + xlo];
             const int pe = C.cstart[row + xhi + 1];
+            if (p >= pe) continue;
+            float4 rn = C.rxyzq[p];
             for (; p < pe; ++p) {
-                const float4 r = C.rxyzq[p];
+                const float4 r = rn;  // next record is requested before this one is used
+                if (p + 1 < pe) rn = C.rxyzq[p + 1];
                 const float dx = (xh - r.x) + xl;
                 const float dy = (yh - r.y) + yl;
                 const float dz = (zh - r.z) + zl;
@@ -145,17 +148,34 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
     }
 }
 
-// One workgroup per pose.  out[2k] = E_raw, out[2k+1] = V, outS[k] = S.
-// PERATOM additionally writes each ligand atom's (E, V) (sorted order) to pa_e/pa_v.
+// XCD-aware tile order: the hardware places workgroup b on XCD b % 8, so give
+// each XCD a contiguous range of tiles (a pose's ligand chunks then share one
+// L2).  Bijective on [0, total).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned total) {
+    const unsigned q = total / 8u, r = total % 8u, x = b % 8u, slot = b / 8u;
+    return x < r ? x * (q + 1u) + slot : r * (q + 1u) + (x - r) * q + slot;
+}
+
+// One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
+// out[2k+1] = V, outS[k] = S.  With several chunks per pose the tiles add their
+// fixed-point partials into acc[3k..3k+2] (integer atomics: exact, order-free);
+// the last tile of a pose (counter cnt[k]) writes the result and re-zeroes acc
+// and cnt for the next launch.  Desolvation needs the whole ligand in one tile.
+// PERATOM also writes each ligand atom's (E, V) (sorted order) to pa_e/pa_v.
 template <bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
 __global__ void __launch_bounds__(512)
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
-             float* __restrict__ pa_v) {
+             float* __restrict__ pa_v, long long* __restrict__ acc, unsigned* __restrict__ cnt,
+             int chunk, int nchunks) {
     extern __shared__ unsigned lds_min[];
     __shared__ long long red[3][16];
-    const int pose = blockIdx.x;
-    if (pose >= N) return;
+    const unsigned total = (unsigned)N * (unsigned)nchunks;
+    if (blockIdx.x >= total) return;
+    const unsigned L = xcd_tile(blockIdx.x, total);
+    const int pose = (int)(L / (unsigned)nchunks);
+    const int j0 = (int)(L % (unsigned)nchunks) * chunk;
+    const int j1 = min(C.nl, j0 + chunk);
     const int tid = threadIdx.x, bs = blockDim.x;
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
 
@@ -170,7 +190,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     }
 
     long long e = 0, v = 0, s = 0;
-    for (int j = tid; j < C.nl; j += bs) {
+    for (int j = j0 + tid; j < j1; j += bs) {
         const float4 lb = C.lxyzq[j];
         const float4 lp = C.lpar[j];
         double x, y, z;
@@ -211,17 +231,27 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         red[2][w] = s;
     }
     __syncthreads();
-    if (tid == 0) {
-        long long te = 0, tv = 0, ts = 0;
-        for (int k = 0; k < nw; ++k) {
-            te += red[0][k];
-            tv += red[1][k];
-            ts += red[2][k];
-        }
-        out[2 * (size_t)pose] = (double)te / E_SCALE;
-        out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
-        if (DES != DES_NONE) outS[pose] = (double)ts / E_SCALE;
+    if (tid != 0) return;
+    long long te = 0, tv = 0, ts = 0;
+    for (int k = 0; k < nw; ++k) {
+        te += red[0][k];
+        tv += red[1][k];
+        ts += red[2][k];
     }
+    if (nchunks > 1) {
+        unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
+        atomicAdd(&a[0], (unsigned long long)te);
+        atomicAdd(&a[1], (unsigned long long)tv);
+        __threadfence();
+        if (atomicAdd(&cnt[pose], 1u) != (unsigned)nchunks - 1u) return;
+        __threadfence();
+        te = (long long)atomicExch(&a[0], 0ull);
+        tv = (long long)atomicExch(&a[1], 0ull);
+        atomicExch(&cnt[pose], 0u);
+    }
+    out[2 * (size_t)pose] = (double)te / E_SCALE;
+    out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
+    if (DES != DES_NONE) outS[pose] = (double)ts / E_SCALE;
 }
 
 // Diagnostic: device transform of every (pose, ligand atom), sorted order.
@@ -247,6 +277,7 @@ struct lkl_ctx {
     hipStream_t stream = nullptr;
     size_t lds_max = 65536;
     int block = 256;
+    int chunk = 0;  // ligand atoms per workgroup tile (0 = the whole ligand)
     float cell_size = 5.0f;
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
@@ -280,6 +311,9 @@ struct lkl_ctx {
     size_t cap_pa = 0, cap_pa_v = 0;
     double* d_xyz = nullptr;
     size_t cap_xyz = 0;
+    long long* d_acc = nullptr;  // per-pose tile accumulators (kept zero between launches)
+    unsigned* d_cnt = nullptr;
+    size_t cap_acc = 0;
 };
 
 template <typename T>
@@ -462,7 +496,8 @@ static void ctx_free(lkl_ctx* c) {
     (void)hipSetDevice(c->device);
     void* bufs[] = {c->d_phi, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
-                    c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz};
+                    c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
+                    c->d_acc, c->d_cnt};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -526,7 +561,21 @@ static lkl_ctx* ctx_create(const lkl_complex_desc* d) {
 
 // Kernel dispatch over the flag word.
 typedef void (*score_fn)(DevComplex, const double*, int, double*, double*, unsigned*, float*,
-                         float*);
+                         float*, long long*, unsigned*, int, int);
+
+// Zeroed per-pose tile accumulators (zeroed once here; every launch leaves them zero).
+static int reserve_acc(lkl_ctx* c, size_t N) {
+    if (N <= c->cap_acc && c->d_acc) return 0;
+    if (c->d_acc) { (void)hipFree(c->d_acc); c->d_acc = nullptr; }
+    if (c->d_cnt) { (void)hipFree(c->d_cnt); c->d_cnt = nullptr; }
+    c->cap_acc = 0;
+    HIPCK("alloc", hipMalloc((void**)&c->d_acc, sizeof(long long) * 3 * N));
+    HIPCK("alloc", hipMalloc((void**)&c->d_cnt, sizeof(unsigned) * N));
+    HIPCK("alloc", hipMemset(c->d_acc, 0, sizeof(long long) * 3 * N));
+    HIPCK("alloc", hipMemset(c->d_cnt, 0, sizeof(unsigned) * N));
+    c->cap_acc = N;
+    return 0;
+}
 
 template <int DES, bool PA>
 static score_fn pick3(unsigned f) {
@@ -576,9 +625,14 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     else if (des == DES_LDS) fn = pick3<DES_LDS, false>(flags);
     else if (des == DES_GLOBAL) fn = pick3<DES_GLOBAL, false>(flags);
     else fn = pick3<DES_NONE, false>(flags);
+    int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
+    const int nchunks = (c->nl + chunk - 1) / chunk;
+    if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
+    if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
     if (fail_at("launch")) { set_err("launch_score stage=launch: injected failure"); return -1; }
-    hipLaunchKernelGGL(fn, dim3((unsigned)N), dim3((unsigned)c->block), shm, st, C, d_poses, N,
-                       d_out, d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v);
+    hipLaunchKernelGGL(fn, dim3((unsigned)(N * nchunks)), dim3((unsigned)c->block), shm, st, C,
+                       d_poses, N, d_out, d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_acc,
+                       c->d_cnt, chunk, nchunks);
     HIPCK("launch", hipGetLastError());
     return 0;
 }
@@ -658,19 +712,97 @@ LKL_API int lkl_ctx_transform(lkl_ctx* c, const double* poses, int N, double* ou
     return 0;
 }
 
-LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block) {
+LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block, int chunk) {
     if (!c) { set_err("lkl_ctx_set_tiles: null context"); return -1; }
     if (!(cell_size > 0.5f && cell_size <= 20.0f)) { set_err("cell_size %.3f out of (0.5, 20]", cell_size); return -1; }
     if (block != 64 && block != 128 && block != 256 && block != 512) {
         set_err("block %d not in {64,128,256,512}", block);
         return -1;
     }
+    if (chunk < 0) { set_err("chunk %d must be >= 0", chunk); return -1; }
     HIPCK("probe", hipSetDevice(c->device));
     c->block = block;
+    c->chunk = chunk;
     if (cell_size != c->cell_size) {
         c->cell_size = cell_size;
         if (build_receptor(c)) return -1;
     }
+    return 0;
+}
+
+// ── autotuner ────────────────────────────────────────────────────────────────
+// Times every (receptor cell edge, workgroup size) candidate on the caller's
+// poses with HIP events and keeps the fastest.  Per-pose results are bitwise
+// identical for every candidate (fixed-point sums), so tuning never changes a
+// score.  table (optional) receives n_cand rows of (cell, block, ms).
+static const float k_cells[] = {3.4f, 4.0f, 5.0f, 6.0f, 7.5f};
+static const int k_blocks[] = {64, 128, 256, 512};
+static const int k_chunk_mult[] = {1, 2, 4, 0};  // ligand atoms per lane (0 = whole ligand)
+
+LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned flags,
+                             float* best_cell, int* best_block, int* best_chunk, float* table,
+                             int* n_table) {
+    if (!c || !poses || N <= 0) { set_err("lkl_ctx_autotune: bad arguments"); return -1; }
+    if (validate_flags(c, flags)) return -1;
+    HIPCK("probe", hipSetDevice(c->device));
+    if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
+    if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
+    if (dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
+    HIPCK("upload", hipMemcpy(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPCK("alloc", hipEventCreate(&e0));
+    HIPCK("alloc", hipEventCreate(&e1));
+    float bc = c->cell_size, bms = INFINITY;
+    int bb = c->block, bch = c->chunk, k = 0;
+    int rc = 0;
+    for (float cell : k_cells) {
+        if (cell != c->cell_size) {
+            c->cell_size = cell;
+            if (build_receptor(c)) { rc = -1; break; }
+        }
+        for (int block : k_blocks) {
+            for (int mult : k_chunk_mult) {
+                const int chunk = mult ? block * mult : 0;
+                if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile
+                if ((flags & F_DESOLV) && mult) continue;
+                c->block = block;
+                c->chunk = chunk;
+                double* dS = (flags & F_DESOLV) ? c->d_outS : nullptr;
+                if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) { rc = -1; break; }
+                float ms[3];
+                for (int r = 0; r < 3 && rc == 0; ++r) {
+                    (void)hipEventRecord(e0, c->stream);
+                    if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) rc = -1;
+                    (void)hipEventRecord(e1, c->stream);
+                    (void)hipEventSynchronize(e1);
+                    (void)hipEventElapsedTime(&ms[r], e0, e1);
+                }
+                if (rc) break;
+                std::sort(ms, ms + 3);
+                if (table && k < 128) {
+                    table[4 * k] = cell; table[4 * k + 1] = (float)block;
+                    table[4 * k + 2] = (float)chunk; table[4 * k + 3] = ms[1];
+                }
+                ++k;
+                if (ms[1] < bms) { bms = ms[1]; bc = cell; bb = block; bch = chunk; }
+            }
+            if (rc) break;
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return -1;
+    if (n_table) *n_table = std::min(k, 128);
+    c->block = bb;
+    c->chunk = bch;
+    if (bc != c->cell_size) {
+        c->cell_size = bc;
+        if (build_receptor(c)) return -1;
+    }
+    if (best_cell) *best_cell = bc;
+    if (best_block) *best_block = bb;
+    if (best_chunk) *best_chunk = bch;
     return 0;
 }
 

@@ -119,6 +119,17 @@ class Context:
                                             _capi.ptr(out, C.c_double)), "lkl_ctx_transform")
         return out
 
-    def set_tiles(self, cell_size, block):
-        _capi.check(lib().lkl_ctx_set_tiles(self.h, float(cell_size), int(block)),
+    def autotune(self, poses, flags):
+        """lkl_ctx_autotune -> (cell, block, chunk, table[(cell, block, chunk, ms)])."""
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        cell, block, chunk, n = C.c_float(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        table = np.zeros((128, 4), np.float32)
+        _capi.check(lib().lkl_ctx_autotune(self.h, _capi.ptr(p, C.c_double), p.shape[0], flags,
+                                           C.byref(cell), C.byref(block), C.byref(chunk),
+                                           _capi.ptr(table, C.c_float), C.byref(n)),
+                    "lkl_ctx_autotune")
+        return cell.value, block.value, chunk.value, table[:n.value].tolist()
+
+    def set_tiles(self, cell_size, block, chunk=0):
+        _capi.check(lib().lkl_ctx_set_tiles(self.h, float(cell_size), int(block), int(chunk)),
                     "lkl_ctx_set_tiles")

@@ -19,6 +19,10 @@ _u8 = C.POINTER(C.c_ubyte)
 _i = C.POINTER(C.c_int)
 
 
+SCORE_FALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int,
+                             C.POINTER(C.c_double))
+
+
 class ComplexDesc(C.Structure):
     """lkl_complex_desc."""
     _fields_ = [
@@ -61,7 +65,8 @@ def lib():
         "lkl_ctx_score_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
         "lkl_ctx_transform": (C.c_int, [C.c_void_p, _d, C.c_int, _d]),
-        "lkl_ctx_set_tiles": (C.c_int, [C.c_void_p, C.c_float, C.c_int]),
+        "lkl_ctx_set_tiles": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int]),
+        "lkl_ctx_autotune": (C.c_int, [C.c_void_p, _d, C.c_int, C.c_uint, _f, _i, _i, _f, _i]),
         "lkl_batch_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                       C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int,
                                       _i, _i, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
@@ -81,6 +86,20 @@ def lib():
         "lkl_field_build": (C.c_int, [_f, _f, C.c_int, _f]),
         "lkl_synth_rsa": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_uint64, C.c_int64, _d, C.POINTER(C.c_int64)]),
+        "lkl_gso_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, _i, _d, C.c_uint64,
+                                        SCORE_FALLBACK, C.c_void_p]),
+        "lkl_gso_destroy": (None, [C.c_void_p]),
+        "lkl_gso_update_luciferin": (C.c_int, [C.c_void_p]),
+        "lkl_gso_movement_phase": (C.c_int, [C.c_void_p]),
+        "lkl_gso_run": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_gso_size": (C.c_int, [C.c_void_p]),
+        "lkl_gso_state": (C.c_int, [C.c_void_p, _d, _d, _d, _d, _i, _u8]),
+        "lkl_gso_set_poses": (C.c_int, [C.c_void_p, _d, _u8]),
+        "lkl_gso_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                    C.POINTER(C.c_longlong)]),
+        "lkl_gso_save": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p]),
+        "lkl_rng_f64s": (None, [C.c_uint64, C.c_int, _d]),
+        "lkl_quat_slerp": (None, [_d, _d, C.c_double, _d]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -167,6 +167,27 @@ def make_poses(n, rec_axes, seed, shell=15.0):
     return _f32(np.concatenate([q, t], 1))
 
 
+def make_swarm_positions(n_swarms, n_glow, rec_axes, seed, lig_radius=0.0, spread=3.0):
+    """Initial GSO positions, rows (tx, ty, tz, w, x, y, z), f32 values: each
+    swarm centre sits on the receptor surface (uniform direction) pushed out by
+    the ligand radius; glowworms are uniform in a ball of `spread` A around it
+    with uniform random rotations."""
+    rng = np.random.default_rng(seed)
+    rows = []
+    a, b, c = rec_axes
+    for _ in range(n_swarms):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        r_surf = 1.0 / math.sqrt((u[0] / a) ** 2 + (u[1] / b) ** 2 + (u[2] / c) ** 2)
+        centre = u * (r_surf + lig_radius)
+        d = rng.normal(size=(n_glow, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        t = centre + d * (spread * rng.random(n_glow) ** (1.0 / 3.0))[:, None]
+        q = random_quaternions(n_glow, rng)
+        rows.append(np.concatenate([t, q], 1))
+    return _f32(np.concatenate(rows, 0))
+
+
 CONFIGS = {
     # name: (n_rec, rec axes as stated, n_lig, lig radius as stated, seed)
     "c1": (1500, (22.0, 18.0, 16.0), 800, 10.0, 42),

@@ -105,6 +105,19 @@ double orc_dna_energy_exact(const orc_mol* rec, const orc_mol* lig, const double
  * 27-cell enumeration (NearCells with the given cell edge) over a point set. */
 int64_t orc_nearcells_pair_count(const double* xyz, int n, double cell);
 
+/* ── GSO (lkl_oracle_gso.c) ─────────────────────────────────────────────── */
+/* rand 0.7.3 StdRng (ChaCha20, seed_from_u64) draws of gen::<f64>(). */
+void orc_rng_f64s(uint64_t seed, int n, double* out);
+/* qt.rs:106-116 Quaternion::random, n draws from one seeded StdRng. */
+void orc_quat_random(uint64_t seed, int n, double* out);
+/* qt.rs:80-104 */
+void orc_slerp(const double* a, const double* b, double t, double* out);
+/* lib.rs:47-78 GSO over one swarm with CPU-grid scoring of `family`. */
+void orc_gso_run(int family, const orc_mol* rec, const orc_mol* lig, const orc_field* f,
+                 const double* positions, int n, uint64_t seed, int steps, int nthreads,
+                 double* out_pose, double* out_luc, double* out_score, double* out_vision,
+                 int* out_nnb);
+
 #ifdef __cplusplus
 }
 #endif

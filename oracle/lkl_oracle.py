@@ -76,6 +76,16 @@ def lib():
                                            C.POINTER(Field), _d, C.POINTER(Parts)]
         L.orc_dna_energy_exact.restype = C.c_double
         L.orc_dna_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
+        L.orc_rng_f64s.restype = None
+        L.orc_rng_f64s.argtypes = [C.c_uint64, C.c_int, _d]
+        L.orc_quat_random.restype = None
+        L.orc_quat_random.argtypes = [C.c_uint64, C.c_int, _d]
+        L.orc_slerp.restype = None
+        L.orc_slerp.argtypes = [_d, _d, C.c_double, _d]
+        L.orc_gso_run.restype = None
+        L.orc_gso_run.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol), C.POINTER(Field), _d,
+                                  C.c_int, C.c_uint64, C.c_int, C.c_int, _d, _d, _d, _d,
+                                  C.POINTER(C.c_int)]
         L.orc_nearcells_pair_count.restype = C.c_int64
         L.orc_nearcells_pair_count.argtypes = [_d, C.c_int, C.c_double]
         _lib = L
@@ -172,6 +182,19 @@ class Oracle:
                                    self._fp() if method != "vdw" else None, _p(p), C.byref(out))
         return out
 
+    def gso(self, method, positions, seed, steps, nthreads=4):
+        """lib.rs:47-78 with CPU-grid scoring; per-step arrays after each move."""
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 7)
+        n = pos.shape[0]
+        pose = np.zeros((steps, n, 7))
+        luc, sc, vis = np.zeros((steps, n)), np.zeros((steps, n)), np.zeros((steps, n))
+        nnb = np.zeros((steps, n), np.int32)
+        lib().orc_gso_run(FAMILY[method], C.byref(self.rec.s), C.byref(self.lig.s),
+                          self._fp() if method != "vdw" else None, _p(pos), n, int(seed),
+                          int(steps), int(nthreads), _p(pose), _p(luc), _p(sc), _p(vis),
+                          _p(nnb, C.c_int))
+        return {"pose": pose, "luciferin": luc, "scoring": sc, "vision": vis, "neighbors": nnb}
+
     def dna_exact(self, pose):
         p = np.ascontiguousarray(pose, np.float64)
         return lib().orc_dna_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
@@ -186,6 +209,26 @@ class Oracle:
 
 def fold(method, e, v, s=0.0):
     return lib().orc_fold_score(FAMILY[method], e, v, s)
+
+
+def rng_f64s(seed, n):
+    out = np.zeros(n)
+    lib().orc_rng_f64s(int(seed), int(n), _p(out))
+    return out
+
+
+def quat_random(seed, n):
+    out = np.zeros((n, 4))
+    lib().orc_quat_random(int(seed), int(n), _p(out))
+    return out
+
+
+def slerp(a, b, t):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    out = np.zeros(4)
+    lib().orc_slerp(_p(a), _p(b), float(t), _p(out))
+    return out
 
 
 def nearcells_pair_count(xyz, cell):

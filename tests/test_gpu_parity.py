@@ -123,14 +123,15 @@ def test_tile_and_order_invariance(small):
     flags = lklight.FLAGS_DNA
     base = run(ctx, poses, flags)
     base_c = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
-    for cell, block in [(10.0, 256), (3.4, 64), (6.0, 512), (4.0, 128)]:
-        ctx.set_tiles(cell, block)
+    for cell, block, chunk in [(10.0, 256, 0), (3.4, 64, 64), (6.0, 512, 0), (4.0, 128, 256),
+                               (5.0, 64, 128)]:
+        ctx.set_tiles(cell, block, chunk)
         r = run(ctx, poses, flags)
         rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         for k in ("e", "v"):
-            assert np.array_equal(r[k], base[k]), (cell, block, k)
+            assert np.array_equal(r[k], base[k]), (cell, block, chunk, k)
         for k in ("e", "v", "s"):
-            assert np.array_equal(rc[k], base_c[k]), (cell, block, k)
+            assert np.array_equal(rc[k], base_c[k]), (cell, block, chunk, k)
     ctx.close()
     say("order invariance")
     rng = np.random.default_rng(0)
@@ -149,8 +150,9 @@ def test_tile_and_order_invariance(small):
 
 def test_determinism_and_batch_split(small):
     cx, orc, poses = small
-    say("determinism")
+    say("determinism (multi-tile poses: accumulators must re-zero between launches)")
     ctx = lklight.Context(cx)
+    ctx.set_tiles(4.0, 64, 64)
     a = run(ctx, poses, lklight.FLAGS_DNA)
     b = run(ctx, poses, lklight.FLAGS_DNA)
     c = run(ctx, poses[:17], lklight.FLAGS_DNA)

@@ -1,0 +1,151 @@
+"""GSO driver: RNG and quaternion KATs of qt.rs, and trajectories against the
+oracle's restatement of lib.rs / swarm.rs / glowworm.rs."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import lklight
+from lklight import gso as G
+from lklight import synth
+from oracle import lkl_oracle as O
+
+EPS = 2.220446049250313e-16  # f64::EPSILON of qt.rs float_equals
+
+
+def _qeq(a, b):
+    return all(abs(x - y) < EPS for x, y in zip(a, b))
+
+
+def test_rng_kat_product_and_oracle():
+    """qt.rs:474-487: StdRng::seed_from_u64(324324324) -> Quaternion::random."""
+    a, b = G.rng_f64s(324324324, 3), O.rng_f64s(324324324, 3)
+    assert np.array_equal(a, b)
+    u1, u2, u3 = a
+    q = (math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+         math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3))
+    kat = (0.31924330894562036, -0.5980633213833059, 0.5444724265858514, 0.49391674399349367)
+    assert _qeq(q, kat), q
+    assert _qeq(O.quat_random(324324324, 1)[0], kat)
+    # the stream continues across ChaCha blocks identically
+    assert np.array_equal(G.rng_f64s(7, 100), O.rng_f64s(7, 100))
+
+
+@pytest.mark.parametrize("impl", ["product", "oracle"])
+def test_slerp_kats(impl):
+    """qt.rs:424-472."""
+    sl = G.slerp if impl == "product" else O.slerp
+    assert _qeq(sl([1, 0, 0, 2], [3, -1, 4, 3], 0.0), (0.4472135954999579, 0.0, 0.0, 0.8944271909999159))
+    assert _qeq(sl([1, 0, 0, 2], [3, -1, 4, 3], 1.0),
+                (0.50709255283711, -0.1690308509457033, 0.6761234037828132, 0.50709255283711))
+    q = [0.7071067811865476, 0.0, 0.0, 0.7071067811865476]
+    assert _qeq(sl(q, q, 0.1), q)
+    assert _qeq(sl([1, 0, 0, 0], [0, 0, 1, 0], 0.5), (0.7071067811865475, 0.0, 0.7071067811865475, 0.0))
+    assert _qeq(sl([0.7071067811865475, 0, 0, 0.7071067811865475],
+                   [0.0, 0.7071067811865475, 0.7071067811865475, 0.0], 0.5), (0.5, 0.5, 0.5, 0.5))
+
+
+@pytest.fixture(scope="module")
+def small_vdw():
+    cx = synth.make_config("small")
+    pos = synth.make_swarm_positions(1, 40, cx.meta["rec_axes"], 3, lig_radius=cx.meta["lig_axes"][0])
+    return cx, O.Oracle(cx), pos
+
+
+def test_gso_logic_matches_oracle_on_cpu(small_vdw):
+    """Product driver with the caller's CPU scorer (no GPU context: the
+    supports_batch() == false branch) vs the oracle's GSO: same scores, so the
+    trajectories must agree bit for bit (neighbours, draws, moves, slerp)."""
+    cx, orc, pos = small_vdw
+    steps = 12
+
+    def cpu_scores(poses):
+        r = orc.batch("vdw", poses)
+        return -r["v"]
+
+    g = G.GSO(None, "vdw", pos, seed=324324, fallback=cpu_scores)
+    ref = orc.gso("vdw", pos, 324324, steps)
+    for s in range(steps):
+        g.update_luciferin()
+        g.movement_phase()
+        st = g.state()
+        assert np.array_equal(st["pose"], ref["pose"][s]), s
+        assert np.array_equal(st["luciferin"], ref["luciferin"][s]), s
+        assert np.array_equal(st["neighbors"], ref["neighbors"][s]), s
+        assert np.array_equal(st["vision"], ref["vision"][s]), s
+    assert ref["neighbors"].max() > 0 and (ref["pose"][-1] != ref["pose"][0]).any()
+    # one call per step that had glowworms to re-score (moved, or step 0)
+    assert 1 <= g.stats()["fallback_calls"] <= steps and g.stats()["gpu_launches"] == 0
+    g.close()
+
+
+def test_gso_save_format(small_vdw, tmp_path):
+    """swarm.rs:232-271 line format."""
+    cx, orc, pos = small_vdw
+    g = G.GSO(None, "vdw", pos[:3], seed=1, fallback=lambda p: np.arange(p.shape[0], dtype=float))
+    g.run(1)
+    g.save(0, 1, str(tmp_path))
+    lines = open(os.path.join(tmp_path, "gso_1.out")).read().splitlines()
+    assert lines[0] == "#Coordinates  RecID  LigID  Luciferin  Neighbor's number  Vision Range  Scoring"
+    assert len(lines) == 4
+    head, tail = lines[1].split(")")
+    assert len(head.strip("(").split(", ")) == 7 and all("." in x for x in head.split(", "))
+    f = tail.split()
+    assert f[0] == "0" and f[1] == "0" and len(f) == 6
+    st = g.state()
+    assert f[2] == "%.8f" % st["luciferin"][0] and f[5] == "%.8f" % st["scoring"][0]
+    g.close()
+
+
+def test_gso_failure_without_fallback(small_vdw):
+    cx, orc, pos = small_vdw
+    with pytest.raises(lklight.GpuError):
+        G.GSO(None, "vdw", pos, seed=1)
+
+
+@pytest.mark.gpu
+def test_gso_vdw_step_identical_gpu_vs_cpu(gpu):
+    """vdw GSO on the configs[0] complex (1,500-atom receptor, 800-atom ligand,
+    1 swarm x 50 glowworms x 20 steps): GPU-scored trajectory == CPU-grid-scored
+    trajectory at every step; per-pose scores within 1e-5 relative."""
+    cx = synth.make_config("c1")
+    pos = synth.make_swarm_positions(1, 50, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
+    orc = O.Oracle(cx)
+    steps = 20
+    ref = orc.gso("vdw", pos, 42, steps, nthreads=8)
+    ctx = lklight.Context(cx, with_field=False)
+    g = G.GSO(ctx, "vdw", pos, seed=42)
+    for s in range(steps):
+        g.update_luciferin()
+        g.movement_phase()
+        st = g.state()
+        assert np.array_equal(st["pose"], ref["pose"][s]), s
+        assert np.array_equal(st["neighbors"], ref["neighbors"][s]), s
+        err = np.abs(st["scoring"] - ref["scoring"][s]) / np.maximum(np.abs(ref["scoring"][s]), 1.0)
+        assert err.max() <= 1e-5, (s, err.max())
+    stats = g.stats()
+    assert 1 <= stats["gpu_launches"] <= steps and stats["fallback_calls"] == 0
+    g.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gso_dna_multi_swarm_one_launch_per_step(gpu):
+    """Several swarms step together with one scoring launch per step; each
+    swarm's trajectory equals that swarm run alone (swarms are independent)."""
+    cx = synth.make_config("c1")
+    cx.build_field()
+    pos = synth.make_swarm_positions(3, 20, cx.meta["rec_axes"], 5, lig_radius=cx.meta["lig_axes"][0])
+    ctx = lklight.Context(cx)
+    g = G.GSO(ctx, "dna", pos, seed=7, swarm_sizes=[20, 20, 20])
+    g.run(8)
+    both = g.state()
+    assert 1 <= g.stats()["gpu_launches"] <= 8
+    for s in range(3):
+        one = G.GSO(ctx, "dna", pos[20 * s:20 * (s + 1)], seed=7)
+        one.run(8)
+        assert np.array_equal(one.state()["pose"], both["pose"][20 * s:20 * (s + 1)])
+        one.close()
+    g.close()
+    ctx.close()
