@@ -165,6 +165,41 @@ int lkl_ctx_autotune(lkl_ctx* ctx, const double* poses, int N, unsigned flags,
                      float* best_cell, int* best_block, int* best_chunk, float* table,
                      int* n_table);
 
+/* Build the context's far field on the device from its receptor coordinates
+ * and charges (ReceptorField::build, grid_dna.rs:44-149, the reference's values
+ * bit for bit) without a host copy; replaces any grid given at creation. */
+int lkl_ctx_build_field(lkl_ctx* ctx);
+/* Origin and node counts of the context's far field (-1 if it has none). */
+int lkl_ctx_field_info(const lkl_ctx* ctx, double* origin3, int* dims3);
+
+/* ── multi-problem batch (no reference counterpart) ───────────────────────────
+ * Many complexes (contexts) of uneven sizes, each with its own flag word and
+ * pose count, scored by ONE kernel launch per step: one workgroup per
+ * (problem, pose, ligand chunk) tile, largest problems first, the hardware
+ * dispatcher back-filling freed slots.  Every problem's scores are bitwise those of its own
+ * lkl_ctx_score call.  Contexts must outlive the batch; a context re-tiled
+ * later is picked up automatically.  Desolvation is not batched. */
+typedef struct lkl_multi lkl_multi;
+lkl_multi* lkl_multi_create(lkl_ctx* const* ctxs, const unsigned* flags, const int* n_poses,
+                            int n_problems);
+void lkl_multi_destroy(lkl_multi* m);
+/* Total poses P (the batch rows: problem 0's poses first). */
+int lkl_multi_poses(const lkl_multi* m);
+/* Launch shape: block in {64,128,256,512}, ligand atoms per lane (0 = whole
+ * ligand per tile). */
+int lkl_multi_set_tiles(lkl_multi* m, int block, int chunk_mult);
+/* Host buffers poses (P x 7) -> out (P x 2 of E_raw, V).  One step is H2D,
+ * the batch kernel, D2H; with the graph on (default) the three are captured
+ * once into a hipGraph and replayed. */
+int lkl_multi_score(lkl_multi* m, const double* poses, double* out);
+/* Device buffers, one kernel launch on `stream` (NULL = the batch's stream). */
+int lkl_multi_score_async(lkl_multi* m, const double* d_poses, double* d_out, void* stream);
+int lkl_multi_use_graph(lkl_multi* m, int enable);
+/* Autotune (block, chunk_mult) on host poses; table (may be NULL) receives up
+ * to 64 rows (block, chunk_mult, median ms). */
+int lkl_multi_autotune(lkl_multi* m, const double* poses, int* best_block, int* best_mult,
+                       float* table, int* n_table);
+
 /* ── receptor far field and synthetic inputs ──────────────────────────────── */
 
 /* grid_dna.rs:44-61: origin (3 doubles) and node counts (3 ints) of the 0.5 A

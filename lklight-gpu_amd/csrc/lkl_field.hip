@@ -132,6 +132,31 @@ LKL_API long long lkl_field_dims(const float* r_xyz, int nr, double* origin3, in
     return total;
 }
 
+// Build phi into a device buffer d_phi (lkl_field_dims() floats) on `stream`
+// (library-internal: used by lkl_ctx_build_field).  Returns 0 / -1.
+int lkl_field_build_device(const float* r_xyz, const float* r_ele, int nr, float* d_phi,
+                           hipStream_t st) {
+    double o[3];
+    int d[3];
+    if (lkl_field_dims(r_xyz, nr, o, d) <= 0 || !r_ele || !d_phi) return -1;
+    std::vector<float4> at(nr);
+    for (int k = 0; k < nr; ++k)
+        at[k] = make_float4(r_xyz[3 * k], r_xyz[3 * k + 1], r_xyz[3 * k + 2], r_ele[k]);
+    float4* d_at = nullptr;
+    if (hipMalloc((void**)&d_at, sizeof(float4) * (size_t)nr) != hipSuccess) return -1;
+    int rc = 0;
+    if (hipMemcpyAsync(d_at, at.data(), sizeof(float4) * (size_t)nr, hipMemcpyHostToDevice, st) != hipSuccess) rc = -1;
+    if (rc == 0) {
+        dim3 grid((d[0] + BX - 1) / BX, (d[1] + BY - 1) / BY, (d[2] + BZ - 1) / BZ);
+        hipLaunchKernelGGL(field_kernel, grid, dim3(BT), 0, st, d_at, nr, o[0], o[1], o[2], d[0],
+                           d[1], d[2], d_phi);
+        if (hipGetLastError() != hipSuccess) rc = -1;
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) rc = -1;
+    (void)hipFree(d_at);
+    return rc;
+}
+
 // Build phi for a receptor (coordinates + charges) into phi_out (host buffer of
 // lkl_field_dims() floats, x fastest).  Returns 0 / -1.
 LKL_API int lkl_field_build(const float* r_xyz, const float* r_ele, int nr, float* phi_out) {

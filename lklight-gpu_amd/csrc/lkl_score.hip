@@ -254,6 +254,96 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     if (DES != DES_NONE) outS[pose] = (double)ts / E_SCALE;
 }
 
+// ── multi-problem batch ───────────────────────────────────────────────────────
+// Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
+// thread's share (no desolvation, no per-atom output).
+template <bool FAR, bool ELEC, bool CLASH>
+__device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1,
+                                          long long& e, long long& v) {
+    for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
+        const float4 lb = C.lxyzq[j];
+        const float4 lp = C.lpar[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        double minL = MIN_INIT;
+        if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
+        near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
+                                         minL, nullptr);
+    }
+}
+
+struct ProblemDesc {
+    int tile_begin;  // first batch tile of this problem (problems sorted by cost)
+    int nchunks;     // ligand chunks per pose
+    int chunk;       // ligand atoms per chunk
+    int pose_off;    // first pose row of this problem in the batch arrays
+    unsigned flags;
+    int pad[3];
+};
+
+// One launch scores every (problem, pose, ligand chunk) tile of a batch of
+// complexes: one workgroup per tile, tiles numbered largest problem first so
+// the hardware dispatcher starts the expensive work early and back-fills the
+// small problems as workgroup slots free up.  Per-pose partials meet in
+// acc/cnt exactly as in score_kernel, so each problem's results are bitwise
+// those of its own launch.  (No loop around barriers: a claim-loop variant
+// with a divergent tid==0 claim was restructured by the compiler into nested
+// loops that cycle the barriers without the claiming lane, which never ends.)
+__global__ void __launch_bounds__(512)
+multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__ probs, int K,
+             int total_tiles, const double* __restrict__ poses, double* __restrict__ out,
+             long long* __restrict__ acc, unsigned* __restrict__ cnt) {
+    __shared__ long long red[2][16];
+    const int t = (int)blockIdx.x;
+    if (t >= total_tiles) return;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    int lo = 0, hi = K - 1;  // last problem with tile_begin <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (probs[mid].tile_begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const ProblemDesc P = probs[lo];
+    const DevComplex C = cxs[lo];
+    const int local = t - P.tile_begin;
+    const int pose = P.pose_off + local / P.nchunks;
+    const int j0 = (local % P.nchunks) * P.chunk;
+    const int j1 = min(C.nl, j0 + P.chunk);
+    const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    long long e = 0, v = 0;
+    switch (P.flags & 7u) {
+        case 0: tile_sums<false, false, false>(C, M, j0, j1, e, v); break;
+        case 1: tile_sums<true, false, false>(C, M, j0, j1, e, v); break;
+        case 2: tile_sums<false, true, false>(C, M, j0, j1, e, v); break;
+        case 3: tile_sums<true, true, false>(C, M, j0, j1, e, v); break;
+        case 4: tile_sums<false, false, true>(C, M, j0, j1, e, v); break;
+        case 5: tile_sums<true, false, true>(C, M, j0, j1, e, v); break;
+        case 6: tile_sums<false, true, true>(C, M, j0, j1, e, v); break;
+        default: tile_sums<true, true, true>(C, M, j0, j1, e, v); break;
+    }
+    e = wave_sum(e);
+    v = wave_sum(v);
+    const int w = tid >> 6, nw = (bs + 63) >> 6;
+    if ((tid & 63) == 0) { red[0][w] = e; red[1][w] = v; }
+    __syncthreads();
+    if (tid != 0) return;
+    long long te = 0, tv = 0;
+    for (int k = 0; k < nw; ++k) { te += red[0][k]; tv += red[1][k]; }
+    if (P.nchunks > 1) {
+        unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
+        atomicAdd(&a[0], (unsigned long long)te);
+        atomicAdd(&a[1], (unsigned long long)tv);
+        __threadfence();
+        if (atomicAdd(&cnt[pose], 1u) != (unsigned)P.nchunks - 1u) return;
+        __threadfence();
+        te = (long long)atomicExch(&a[0], 0ull);
+        tv = (long long)atomicExch(&a[1], 0ull);
+        atomicExch(&cnt[pose], 0u);
+    }
+    out[2 * (size_t)pose] = (double)te / E_SCALE;
+    out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
+}
+
 // Diagnostic: device transform of every (pose, ligand atom), sorted order.
 __global__ void transform_kernel(DevComplex C, const double* __restrict__ poses, int N,
                                  double* __restrict__ outxyz) {
@@ -314,7 +404,12 @@ struct lkl_ctx {
     long long* d_acc = nullptr;  // per-pose tile accumulators (kept zero between launches)
     unsigned* d_cnt = nullptr;
     size_t cap_acc = 0;
+    unsigned version = 0;  // bumped whenever a device array is replaced
 };
+
+int lkl_field_build_device(const float* r_xyz, const float* r_ele, int nr, float* d_phi,
+                           hipStream_t st);
+extern "C" long long lkl_field_dims(const float* r_xyz, int nr, double* origin3, int* dims3);
 
 template <typename T>
 static int dev_upload(T** dst, const T* src, size_t n) {
@@ -392,6 +487,7 @@ static int build_receptor(lkl_ctx* c) {
         if (dev_upload(&c->d_rdes, des.data(), (size_t)nr)) return -1;
         if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr)) return -1;
     }
+    c->version++;
     c->ncx = n[0];
     c->ncy = n[1];
     c->ncz = n[2];
@@ -452,6 +548,7 @@ static int build_ligand(lkl_ctx* c, const float* xyz, const float* ele, const fl
         if (dev_upload(&c->d_ldes, ld.data(), (size_t)nl)) return -1;
         if (dev_upload(&c->d_lasa, la.data(), (size_t)nl)) return -1;
     }
+    c->version++;
     return 0;
 }
 
@@ -693,6 +790,35 @@ LKL_API int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsi
     return launch_score(ctx, d_poses, N, flags, d_out, d_outS, st, false);
 }
 
+LKL_API int lkl_ctx_build_field(lkl_ctx* c) {
+    if (!c) { set_err("lkl_ctx_build_field: null context"); return -1; }
+    if (c->nr <= 0 || c->r_ele.empty()) { set_err("lkl_ctx_build_field: receptor charges required"); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    double o[3];
+    int d[3];
+    const long long total = lkl_field_dims(c->r_xyz.data(), c->nr, o, d);
+    if (total <= 0) { set_err("lkl_ctx_build_field: bad field box"); return -1; }
+    if (c->d_phi) { (void)hipFree(c->d_phi); c->d_phi = nullptr; }
+    c->has_phi = false;
+    HIPCK("alloc", hipMalloc((void**)&c->d_phi, sizeof(float) * (size_t)total));
+    if (lkl_field_build_device(c->r_xyz.data(), c->r_ele.data(), c->nr, c->d_phi, c->stream)) {
+        set_err("lkl_ctx_build_field: field kernel failed");
+        return -1;
+    }
+    c->nx = d[0]; c->ny = d[1]; c->nz = d[2];
+    c->ox = o[0]; c->oy = o[1]; c->oz = o[2]; c->sp = 0.5;
+    c->has_phi = true;
+    c->version++;
+    return 0;
+}
+
+LKL_API int lkl_ctx_field_info(const lkl_ctx* c, double* origin3, int* dims3) {
+    if (!c || !c->has_phi) return -1;
+    if (origin3) { origin3[0] = c->ox; origin3[1] = c->oy; origin3[2] = c->oz; }
+    if (dims3) { dims3[0] = c->nx; dims3[1] = c->ny; dims3[2] = c->nz; }
+    return 0;
+}
+
 LKL_API int lkl_ctx_transform(lkl_ctx* c, const double* poses, int N, double* out_xyz) {
     if (!c || !poses || !out_xyz || N <= 0) { set_err("lkl_ctx_transform: bad arguments"); return -1; }
     HIPCK("probe", hipSetDevice(c->device));
@@ -803,6 +929,252 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
     if (best_cell) *best_cell = bc;
     if (best_block) *best_block = bb;
     if (best_chunk) *best_chunk = bch;
+    return 0;
+}
+
+// ── batch object ─────────────────────────────────────────────────────────────
+struct lkl_multi {
+    std::vector<lkl_ctx*> ctxs;  // not owned
+    std::vector<unsigned> flags;
+    std::vector<int> npose, pose_off, order;  // order: problems by descending cost
+    std::vector<unsigned> versions;
+    int K = 0, P = 0, total_tiles = 0, device = 0, cus = 256;
+    int block = 256, chunk_mult = 2;
+    hipStream_t stream = nullptr;
+    DevComplex* d_cx = nullptr;
+    ProblemDesc* d_prob = nullptr;
+    double *d_poses = nullptr, *d_out = nullptr;
+    long long* d_acc = nullptr;
+    unsigned* d_cnt = nullptr;
+    double *h_poses = nullptr, *h_out = nullptr;  // pinned staging
+    bool use_graph = true, graph_ok = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_drop_graph(lkl_multi* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_ok = false;
+}
+
+static void batch_free(lkl_multi* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    batch_drop_graph(b);
+    void* bufs[] = {b->d_cx, b->d_prob, b->d_poses, b->d_out, b->d_acc, b->d_cnt};
+    for (void* p : bufs)
+        if (p) (void)hipFree(p);
+    if (b->h_poses) (void)hipHostFree(b->h_poses);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+// (Re)build the problem descriptors from the contexts' current layouts.
+static int batch_refresh(lkl_multi* b) {
+    std::vector<DevComplex> cx(b->K);
+    std::vector<ProblemDesc> pd(b->K);
+    int tile = 0;
+    for (int r = 0; r < b->K; ++r) {
+        const int k = b->order[r];
+        lkl_ctx* c = b->ctxs[k];
+        cx[r] = dev_view(c);
+        const int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
+        pd[r].tile_begin = tile;
+        pd[r].nchunks = (c->nl + chunk - 1) / chunk;
+        pd[r].chunk = chunk;
+        pd[r].pose_off = b->pose_off[k];
+        pd[r].flags = b->flags[k];
+        tile += b->npose[k] * pd[r].nchunks;
+        b->versions[k] = c->version;
+    }
+    b->total_tiles = tile;
+    HIPCK("upload", hipMemcpy(b->d_cx, cx.data(), sizeof(DevComplex) * b->K, hipMemcpyHostToDevice));
+    HIPCK("upload", hipMemcpy(b->d_prob, pd.data(), sizeof(ProblemDesc) * b->K, hipMemcpyHostToDevice));
+    batch_drop_graph(b);
+    return 0;
+}
+
+static int batch_check(lkl_multi* b) {
+    for (int k = 0; k < b->K; ++k)
+        if (b->versions[k] != b->ctxs[k]->version) return batch_refresh(b);
+    return 0;
+}
+
+static int batch_launch(lkl_multi* b, const double* d_poses, double* d_out, hipStream_t st) {
+    if (b->total_tiles <= 0) return 0;
+    hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), 0, st,
+                       b->d_cx, b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt);
+    HIPCK("launch", hipGetLastError());
+    return 0;
+}
+
+LKL_API lkl_multi* lkl_multi_create(lkl_ctx* const* ctxs, const unsigned* flags,
+                                    const int* n_poses, int n_problems) {
+    if (!ctxs || !flags || !n_poses || n_problems <= 0) { set_err("lkl_multi_create: bad arguments"); return nullptr; }
+    lkl_multi* b = new lkl_multi();
+    auto fail = [&]() -> lkl_multi* { batch_free(b); return nullptr; };
+    b->K = n_problems;
+    double cost_total = 0;
+    std::vector<double> cost(n_problems);
+    for (int k = 0; k < n_problems; ++k) {
+        lkl_ctx* c = ctxs[k];
+        if (!c || n_poses[k] <= 0) { set_err("lkl_multi_create: problem %d has no context or poses", k); return fail(); }
+        if (flags[k] & F_DESOLV) { set_err("lkl_multi_create: desolvation is not batched (problem %d)", k); return fail(); }
+        if (validate_flags(c, flags[k])) return fail();
+        b->ctxs.push_back(c);
+        b->flags.push_back(flags[k]);
+        b->npose.push_back(n_poses[k]);
+        b->pose_off.push_back(b->P);
+        b->P += n_poses[k];
+        cost[k] = (double)n_poses[k] * c->nl * std::max(1, c->nr);
+        cost_total += cost[k];
+    }
+    b->order.resize(n_problems);
+    std::iota(b->order.begin(), b->order.end(), 0);
+    std::stable_sort(b->order.begin(), b->order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+    b->versions.assign(n_problems, ~0u);
+    b->device = ctxs[0]->device;
+    if (hipSetDevice(b->device) != hipSuccess) { set_err("hipSetDevice failed"); return fail(); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess) b->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { set_err("stream"); return fail(); }
+    const size_t P = (size_t)b->P;
+    if (hipMalloc((void**)&b->d_cx, sizeof(DevComplex) * b->K) != hipSuccess ||
+        hipMalloc((void**)&b->d_prob, sizeof(ProblemDesc) * b->K) != hipSuccess ||
+        hipMalloc((void**)&b->d_poses, sizeof(double) * 7 * P) != hipSuccess ||
+        hipMalloc((void**)&b->d_out, sizeof(double) * 2 * P) != hipSuccess ||
+        hipMalloc((void**)&b->d_acc, sizeof(long long) * 3 * P) != hipSuccess ||
+        hipMalloc((void**)&b->d_cnt, sizeof(unsigned) * P) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_poses, sizeof(double) * 7 * P, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_out, sizeof(double) * 2 * P, hipHostMallocDefault) != hipSuccess) {
+        set_err("lkl_multi_create: allocation failed (%zu poses)", P);
+        return fail();
+    }
+    if (hipMemset(b->d_acc, 0, sizeof(long long) * 3 * P) != hipSuccess ||
+        hipMemset(b->d_cnt, 0, sizeof(unsigned) * P) != hipSuccess) {
+        set_err("lkl_multi_create: clearing counters failed");
+        return fail();
+    }
+    if (batch_refresh(b)) return fail();
+    return b;
+}
+
+LKL_API void lkl_multi_destroy(lkl_multi* b) { batch_free(b); }
+
+LKL_API int lkl_multi_poses(const lkl_multi* b) { return b ? b->P : -1; }
+
+LKL_API int lkl_multi_set_tiles(lkl_multi* b, int block, int chunk_mult) {
+    if (!b) { set_err("lkl_multi_set_tiles: null batch"); return -1; }
+    if ((block != 64 && block != 128 && block != 256 && block != 512) || chunk_mult < 0) {
+        set_err("lkl_multi_set_tiles: block %d / chunk_mult %d out of range", block, chunk_mult);
+        return -1;
+    }
+    HIPCK("probe", hipSetDevice(b->device));
+    b->block = block;
+    b->chunk_mult = chunk_mult;
+    return batch_refresh(b);
+}
+
+LKL_API int lkl_multi_use_graph(lkl_multi* b, int enable) {
+    if (!b) return -1;
+    b->use_graph = enable != 0;
+    return 0;
+}
+
+// Device buffers: poses (P x 7, batch order), out (P x 2).  One kernel launch.
+LKL_API int lkl_multi_score_async(lkl_multi* b, const double* d_poses, double* d_out, void* stream) {
+    if (!b || !d_poses || !d_out) { set_err("lkl_multi_score_async: bad arguments"); return -1; }
+    HIPCK("probe", hipSetDevice(b->device));
+    if (batch_check(b)) return -1;
+    return batch_launch(b, d_poses, d_out, stream ? (hipStream_t)stream : b->stream);
+}
+
+// Host buffers: one step = H2D poses, the batch kernel, D2H scores.  With
+// the graph enabled the three are captured once into a hipGraph and replayed.
+LKL_API int lkl_multi_score(lkl_multi* b, const double* poses, double* out) {
+    if (!b || !poses || !out) { set_err("lkl_multi_score: bad arguments"); return -1; }
+    HIPCK("probe", hipSetDevice(b->device));
+    if (batch_check(b)) return -1;
+    const size_t P = (size_t)b->P;
+    memcpy(b->h_poses, poses, sizeof(double) * 7 * P);
+    if (b->use_graph && !b->graph_ok) {
+        HIPCK("capture", hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = hipMemcpyAsync(b->d_poses, b->h_poses, sizeof(double) * 7 * P,
+                                       hipMemcpyHostToDevice, b->stream);
+        if (b->total_tiles > 0)
+            hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), 0,
+                               b->stream, b->d_cx, b->d_prob, b->K, b->total_tiles, b->d_poses,
+                               b->d_out, b->d_acc, b->d_cnt);
+        hipError_t e2 = hipGetLastError();
+        hipError_t e3 = hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 2 * P,
+                                       hipMemcpyDeviceToHost, b->stream);
+        hipError_t e4 = hipStreamEndCapture(b->stream, &b->graph);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+            set_err("lkl_multi_score: graph capture failed");
+            batch_drop_graph(b);
+            return -1;
+        }
+        HIPCK("capture", hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+        b->graph_ok = true;
+    }
+    if (b->use_graph) {
+        HIPCK("launch", hipGraphLaunch(b->exec, b->stream));
+    } else {
+        HIPCK("upload", hipMemcpyAsync(b->d_poses, b->h_poses, sizeof(double) * 7 * P,
+                                       hipMemcpyHostToDevice, b->stream));
+        if (batch_launch(b, b->d_poses, b->d_out, b->stream)) return -1;
+        HIPCK("readback", hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 2 * P,
+                                         hipMemcpyDeviceToHost, b->stream));
+    }
+    HIPCK("sync", hipStreamSynchronize(b->stream));
+    memcpy(out, b->h_out, sizeof(double) * 2 * P);
+    return 0;
+}
+
+// Autotuner of the batch launch shape (block, ligand atoms per lane) on host
+// poses; results are bitwise shape-invariant.
+LKL_API int lkl_multi_autotune(lkl_multi* b, const double* poses, int* best_block, int* best_mult,
+                               float* table, int* n_table) {
+    if (!b || !poses) { set_err("lkl_multi_autotune: bad arguments"); return -1; }
+    HIPCK("probe", hipSetDevice(b->device));
+    HIPCK("upload", hipMemcpy(b->d_poses, poses, sizeof(double) * 7 * (size_t)b->P, hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    HIPCK("alloc", hipEventCreate(&e0));
+    HIPCK("alloc", hipEventCreate(&e1));
+    int bb = b->block, bm = b->chunk_mult, k = 0, rc = 0;
+    float bms = INFINITY;
+    for (int block : k_blocks)
+        for (int mult : {1, 2, 4, 0}) {
+            if (rc) break;
+            b->block = block; b->chunk_mult = mult;
+            if (batch_refresh(b) || batch_launch(b, b->d_poses, b->d_out, b->stream)) { rc = -1; break; }
+            float ms[3];
+            for (int r = 0; r < 3 && rc == 0; ++r) {
+                (void)hipEventRecord(e0, b->stream);
+                if (batch_launch(b, b->d_poses, b->d_out, b->stream)) rc = -1;
+                (void)hipEventRecord(e1, b->stream);
+                (void)hipEventSynchronize(e1);
+                (void)hipEventElapsedTime(&ms[r], e0, e1);
+            }
+            std::sort(ms, ms + 3);
+            if (table && k < 64) {
+                table[3 * k] = (float)block; table[3 * k + 1] = (float)mult; table[3 * k + 2] = ms[1];
+            }
+            ++k;
+            if (ms[1] < bms) { bms = ms[1]; bb = block; bm = mult; }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return -1;
+    b->block = bb; b->chunk_mult = bm;
+    if (batch_refresh(b)) return -1;
+    if (best_block) *best_block = bb;
+    if (best_mult) *best_mult = bm;
+    if (n_table) *n_table = std::min(k, 64);
     return 0;
 }
 

@@ -14,7 +14,7 @@ from ._capi import (F_CLASH, F_DESOLV, F_ELEC, F_FAR, FLAGS_DNA, FLAGS_PYDOCK, F
 
 FACTOR, EPSILON = 332.0, 4.0
 
-__all__ = ["Context", "device_available", "lib", "last_error", "GpuError", "LibraryMissing",
+__all__ = ["Context", "Multi", "device_available", "lib", "last_error", "GpuError", "LibraryMissing",
            "family_flags", "fold_scores", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
            "FLAGS_DNA", "FLAGS_PYDOCK", "FLAGS_VDW"]
 
@@ -42,7 +42,7 @@ def fold_scores(method: str, e_raw, v, s=None):
 class Context:
     """One uploaded complex (lkl_ctx).  Host arrays are copied at creation."""
 
-    def __init__(self, cx, with_field=True, desolv=False, cell_size=0.0):
+    def __init__(self, cx, with_field=True, desolv=False, cell_size=0.0, device_field=False):
         L = lib()
         self._keep = []
         r, l = cx.rec, cx.lig
@@ -81,6 +81,16 @@ class Context:
             raise GpuError(f"lkl_ctx_create failed: {last_error()}")
         self.h = h
         self.nl, self.nr = l.n, r.n
+        if device_field:
+            self.build_field()
+
+    def build_field(self):
+        """lkl_ctx_build_field: phi built on the device, no host copy."""
+        _capi.check(lib().lkl_ctx_build_field(self.h), "lkl_ctx_build_field")
+        o = np.zeros(3)
+        d = np.zeros(3, np.int32)
+        lib().lkl_ctx_field_info(self.h, _capi.ptr(o, C.c_double), _capi.ptr(d, C.c_int))
+        self.field_origin, self.field_dims = tuple(o), tuple(int(v) for v in d)
 
     def close(self):
         if getattr(self, "h", None):
@@ -133,3 +143,57 @@ class Context:
     def set_tiles(self, cell_size, block, chunk=0):
         _capi.check(lib().lkl_ctx_set_tiles(self.h, float(cell_size), int(block), int(chunk)),
                     "lkl_ctx_set_tiles")
+
+
+class Multi:
+    """Many complexes scored by one kernel launch per step (lkl_multi_*)."""
+
+    def __init__(self, ctxs, flags, n_poses):
+        K = len(ctxs)
+        hs = (C.c_void_p * K)(*[c.h for c in ctxs])
+        fl = (C.c_uint * K)(*[int(f) for f in flags])
+        npz = np.ascontiguousarray(n_poses, np.int32)
+        h = lib().lkl_multi_create(hs, fl, _capi.ptr(npz, C.c_int), K)
+        if not h:
+            raise GpuError(f"lkl_multi_create failed: {last_error()}")
+        self.h, self.ctxs, self.P = h, list(ctxs), int(npz.sum())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().lkl_multi_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def score(self, poses):
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        assert p.shape[0] == self.P
+        out = np.zeros((self.P, 2))
+        _capi.check(lib().lkl_multi_score(self.h, _capi.ptr(p, C.c_double), _capi.ptr(out, C.c_double)),
+                    "lkl_multi_score")
+        return out
+
+    def score_async(self, d_poses, d_out, stream=None):
+        _capi.check(lib().lkl_multi_score_async(self.h, C.c_void_p(d_poses), C.c_void_p(d_out),
+                                                C.c_void_p(stream) if stream else None),
+                    "lkl_multi_score_async")
+
+    def use_graph(self, on):
+        lib().lkl_multi_use_graph(self.h, int(bool(on)))
+
+    def set_tiles(self, block, chunk_mult):
+        _capi.check(lib().lkl_multi_set_tiles(self.h, int(block), int(chunk_mult)),
+                    "lkl_multi_set_tiles")
+
+    def autotune(self, poses):
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        b, m, n = C.c_int(), C.c_int(), C.c_int()
+        table = np.zeros((64, 3), np.float32)
+        _capi.check(lib().lkl_multi_autotune(self.h, _capi.ptr(p, C.c_double), C.byref(b), C.byref(m),
+                                             _capi.ptr(table, C.c_float), C.byref(n)),
+                    "lkl_multi_autotune")
+        return b.value, m.value, table[:n.value].tolist()

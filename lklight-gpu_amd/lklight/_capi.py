@@ -86,6 +86,16 @@ def lib():
         "lkl_field_build": (C.c_int, [_f, _f, C.c_int, _f]),
         "lkl_synth_rsa": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_uint64, C.c_int64, _d, C.POINTER(C.c_int64)]),
+        "lkl_ctx_build_field": (C.c_int, [C.c_void_p]),
+        "lkl_ctx_field_info": (C.c_int, [C.c_void_p, _d, _i]),
+        "lkl_multi_create": (C.c_void_p, [C.POINTER(C.c_void_p), C.POINTER(C.c_uint), _i, C.c_int]),
+        "lkl_multi_destroy": (None, [C.c_void_p]),
+        "lkl_multi_poses": (C.c_int, [C.c_void_p]),
+        "lkl_multi_set_tiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "lkl_multi_score": (C.c_int, [C.c_void_p, _d, _d]),
+        "lkl_multi_score_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lkl_multi_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_multi_autotune": (C.c_int, [C.c_void_p, _d, _i, _i, _f, _i]),
         "lkl_gso_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, _i, _d, C.c_uint64,
                                         SCORE_FALLBACK, C.c_void_p]),
         "lkl_gso_destroy": (None, [C.c_void_p]),

@@ -131,14 +131,8 @@ def make_mol(n, axes, seed, net_charge, radii=None, center=False):
     else:
         rad = _f32(rng.uniform(radii[0], radii[1], n))
     # exposure proxy for the reference SASA (non-negative), from the neighbour count
-    d2 = ((xyz[:, None, :] - xyz[None, :, :]) ** 2).sum(-1) if n <= 4000 else None
-    if d2 is not None:
-        nb = (d2 < 64.0).sum(1) - 1
-    else:
-        nb = np.zeros(n)
-        for s in range(0, n, 2000):
-            blk = ((xyz[s:s + 2000, None, :] - xyz[None, :, :]) ** 2).sum(-1)
-            nb[s:s + 2000] = (blk < 64.0).sum(1) - 1
+    from scipy.spatial import cKDTree
+    nb = np.array([len(v) - 1 for v in cKDTree(xyz).query_ball_point(xyz, 8.0)])
     area = 4.0 * math.pi * (rad + 1.4) ** 2 * np.clip(1.0 - nb / 30.0, 0.0, 1.0)
     mol = Mol(xyz=xyz, ele=_f32(q), rad=rad, seps=TYPE_SQRT_EPS[t].astype(np.float64),
               heavy=np.ones(n, np.uint8), hyd=np.ones(n, np.int32),
@@ -214,3 +208,26 @@ def make_config(name, seed=None, radii=None):
     meta = {"config": name, "seed": seed, "rec_axes": raxes, "lig_axes": laxes,
             "rec_rsa_attempts": ra, "lig_rsa_attempts": la, "density": RHO}
     return Complex(rec=rec, lig=lig, meta=meta)
+
+
+def make_config4(n_complex=64, seed0=0, scale=1.0):
+    """64 independent complexes: receptor atoms log-uniform 2,000-20,000, ligand
+    atoms log-uniform 500-5,000 (spheres at the benchmark density), glowworm
+    counts uniform 100-500, families drawn from dna/pydock/vdw; complex k uses
+    seed seed0 + k.  `scale` shrinks every size (tests)."""
+    out = []
+    for k in range(n_complex):
+        rng = np.random.default_rng(seed0 + k)
+        nr = max(50, int(round(math.exp(rng.uniform(math.log(2000), math.log(20000))) * scale)))
+        nl = max(20, int(round(math.exp(rng.uniform(math.log(500), math.log(5000))) * scale)))
+        ng = max(4, int(round(rng.integers(100, 501) * scale)))
+        fam = ["dna", "pydock", "vdw"][int(rng.integers(0, 3))]
+        rr = (3 * nr / (4 * math.pi * RHO)) ** (1 / 3)
+        lr = (3 * nl / (4 * math.pi * RHO)) ** (1 / 3)
+        rec, _ = make_mol(nr, (rr, rr, rr), seed0 + k, +10.0)
+        lig, _ = make_mol(nl, (lr, lr, lr), seed0 + k + 1000, -10.0, center=True)
+        cx = Complex(rec=rec, lig=lig, meta={"config": "c4", "index": k, "family": fam,
+                                             "glowworms": ng, "rec_axes": (rr, rr, rr),
+                                             "lig_axes": (lr, lr, lr), "seed": seed0 + k})
+        out.append(cx)
+    return out
