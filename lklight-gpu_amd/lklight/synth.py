@@ -40,6 +40,13 @@ class Mol:
     hyd: np.ndarray    # i32, 1 = heavy (cpydock model field)
     des: np.ndarray
     asa: np.ndarray
+    typ: np.ndarray = None  # type index (0..19) of the 20-entry table
+
+    @property
+    def group(self):
+        """Type group (0..15) of the surrogate's pair histograms."""
+        t = self.typ if self.typ is not None else np.zeros(self.n, np.int64)
+        return (np.asarray(t) % 16).astype(np.uint8)
 
     @property
     def n(self):
@@ -136,7 +143,7 @@ def make_mol(n, axes, seed, net_charge, radii=None, center=False):
     area = 4.0 * math.pi * (rad + 1.4) ** 2 * np.clip(1.0 - nb / 30.0, 0.0, 1.0)
     mol = Mol(xyz=xyz, ele=_f32(q), rad=rad, seps=TYPE_SQRT_EPS[t].astype(np.float64),
               heavy=np.ones(n, np.uint8), hyd=np.ones(n, np.int32),
-              des=ASP[t % 10].astype(np.float64), asa=_f32(area))
+              des=ASP[t % 10].astype(np.float64), asa=_f32(area), typ=t.astype(np.int32))
     return mol, attempts
 
 
