@@ -264,6 +264,76 @@ void lkl_rng_f64s(unsigned long long seed, int n, double* out);
 /* Quaternion::slerp (qt.rs:80-104) of (w, x, y, z) quaternions. */
 void lkl_quat_slerp(const double* a, const double* b, double t, double* out);
 
+/* ── learned surrogate (no reference counterpart: the screening step that
+ * ranks candidate moves before exact scoring) ────────────────────────────────
+ * Features per pose: 16 receptor type groups x 16 ligand type groups x 24
+ * distance bins (1.25 A, 0-30 A) of atom-pair counts, index
+ * (g_rec * 16 + g_lig) * 24 + bin, log1p-scaled to bf16 (6,144 values).  The
+ * pair distance is f32: x = f32(R(q) v + t) (f64 transform of the scoring
+ * path), d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz)); counted when d2 < 900,
+ * bin = largest b with 1.5625 b^2 <= d2. */
+typedef struct lkl_feat lkl_feat;
+/* Receptor/ligand f32 coordinates (n x 3, ligand in its reference frame) and
+ * type groups (modulo 16).  NULL on error (lkl_last_error). */
+lkl_feat* lkl_feat_create(const float* r_xyz, const unsigned char* r_group, int nr,
+                          const float* l_xyz, const unsigned char* l_group, int nl);
+void lkl_feat_destroy(lkl_feat* f);
+/* Grid mode: the receptor's per-(group, bin) counts are tabulated on nodes of
+ * spacing h (0.25, 0.5, 1 or 2 A) over the receptor box grown by 30 A + h (u8
+ * per count, 384 bytes per node; creation fails if a count exceeds 255), and a
+ * pose's features sum, per ligand atom, the counts of its nearest node
+ * (ix = floor(fmaf(f32(x) - ox, 1/h, 0.5)) per axis).  Requires nr < 65,536
+ * and nl <= 16,384. */
+lkl_feat* lkl_feat_create_grid(const float* r_xyz, const unsigned char* r_group, int nr,
+                               const float* l_xyz, const unsigned char* l_group, int nl,
+                               double spacing);
+/* Origin, node counts and spacing of a grid-mode object; returns the node
+ * count (0 for an exact-mode object, -1 on error). */
+long long lkl_feat_grid_info(const lkl_feat* f, double* origin3, int* dims3, double* spacing);
+/* Copy nodes [first, first + count) of the count table (384 bytes per node,
+ * order g_rec * 24 + bin, node index (iz * ny + iy) * nx + ix).  0 / -1. */
+int lkl_feat_grid_read(const lkl_feat* f, long long first, long long count, unsigned char* out);
+/* Device buffers: poses N x 7 (w, x, y, z, tx, ty, tz) -> bf16 bits N x 6144
+ * and/or u32 counts N x 6144 (either may be NULL), enqueued on `stream`
+ * (hipStream_t; NULL = the object's stream).  0 / -1. */
+int lkl_feat_compute(lkl_feat* f, const double* d_poses, int N, unsigned short* d_out,
+                     unsigned* d_counts, void* stream);
+/* Host buffers, blocking.  0 / -1. */
+int lkl_feat_compute_host(lkl_feat* f, const double* poses, int N, unsigned short* out,
+                          unsigned* counts);
+
+/* MLP 6144 -> 512 -> 512 -> 512 -> 1 with ReLU; bf16 weights/activations,
+ * fp32 accumulation on the matrix cores, fp32 master weights and Adam state.
+ * Flat parameter layout (fp32): W1 [6144][512] (input-major), b1 [512],
+ * W2 [512][512], b2, W3 [512][512], b3, W4 [512], b4 [1]: 3,672,065 values. */
+typedef struct lkl_mlp lkl_mlp;
+long long lkl_mlp_param_count(void);
+/* He-uniform weights from a splitmix64 stream of `seed`, zero biases. */
+lkl_mlp* lkl_mlp_create(unsigned long long seed);
+void lkl_mlp_destroy(lkl_mlp* n);
+int lkl_mlp_get_params(lkl_mlp* n, float* out);
+int lkl_mlp_set_params(lkl_mlp* n, const float* in);
+int lkl_mlp_get_grads(lkl_mlp* n, float* out);
+/* Device buffers: x (B x 6144 bf16 bits) -> y (B fp32) on `stream` (NULL = the
+ * object's stream).  B must be a positive multiple of 128.  0 / -1. */
+int lkl_mlp_forward(lkl_mlp* n, const void* d_x, int B, float* d_y, void* stream);
+/* Loss 0.5 * mean((y - t)^2) and its gradient into the object's gradient
+ * buffer (device x, t; B multiple of 128); *loss_out (host, may be NULL)
+ * receives the loss after a synchronize.  0 / -1. */
+int lkl_mlp_grad(lkl_mlp* n, const void* d_x, const float* d_t, int B, float* loss_out);
+/* One bias-corrected Adam step on the stored gradient.  0 / -1. */
+int lkl_mlp_adam(lkl_mlp* n, float lr, float beta1, float beta2, float eps);
+int lkl_mlp_sync(lkl_mlp* n);
+/* Host-buffer variants (blocking). */
+int lkl_mlp_forward_host(lkl_mlp* n, const unsigned short* x, int B, float* y);
+int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* t, int B, float* loss);
+/* C (M x N fp32) = sum_k A(m, k) B(k, n) on the matrix cores, bf16 operands
+ * given as bits with element strides A(m, k) = A[m*sam + k*sak],
+ * B(k, n) = B[k*sbk + n*sbn]; M, N multiples of 128, K of 32.  0 / -1. */
+int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, long sak,
+                       const unsigned short* B, long b_len, long sbk, long sbn, int M, int N,
+                       int K, float* C);
+
 #ifdef __cplusplus
 }
 #endif

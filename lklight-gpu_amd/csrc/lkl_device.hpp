@@ -178,4 +178,12 @@ __device__ __forceinline__ long long wave_sum(long long x) {
     return x;
 }
 
+// XCD-aware tile order: the hardware places workgroup b on XCD b % 8, so give
+// each XCD a contiguous range of tiles (neighbouring tiles then share one L2).
+// Bijective on [0, total).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned total) {
+    const unsigned q = total / 8u, r = total % 8u, x = b % 8u, slot = b / 8u;
+    return x < r ? x * (q + 1u) + slot : r * (q + 1u) + (x - r) * q + slot;
+}
+
 }  // namespace lkl

@@ -46,6 +46,10 @@ static void set_err(const char* fmt, ...) {
     fprintf(stderr, "[lklight-gpu] %s\n", buf);
 }
 
+// Library-internal: lets the other translation units report through
+// lkl_last_error().
+void lkl_set_error_text(const char* msg) { set_err("%s", msg); }
+
 // LKL_FAIL_AT=<stage> makes the named stage of every call fail, so host-only
 // tests can drive each failure path (stages: probe, alloc, upload, launch,
 // sync, readback).
@@ -148,13 +152,6 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
     }
 }
 
-// XCD-aware tile order: the hardware places workgroup b on XCD b % 8, so give
-// each XCD a contiguous range of tiles (a pose's ligand chunks then share one
-// L2).  Bijective on [0, total).
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned total) {
-    const unsigned q = total / 8u, r = total % 8u, x = b % 8u, slot = b / 8u;
-    return x < r ? x * (q + 1u) + slot : r * (q + 1u) + (x - r) * q + slot;
-}
 
 // One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
 // out[2k+1] = V, outS[k] = S.  With several chunks per pose the tiles add their

@@ -17,6 +17,8 @@ _f = C.POINTER(C.c_float)
 _d = C.POINTER(C.c_double)
 _u8 = C.POINTER(C.c_ubyte)
 _i = C.POINTER(C.c_int)
+_u16 = C.POINTER(C.c_ushort)
+_u32 = C.POINTER(C.c_uint)
 
 
 SCORE_FALLBACK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.c_int,
@@ -108,6 +110,28 @@ def lib():
         "lkl_gso_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)]),
         "lkl_gso_save": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p]),
+        "lkl_feat_create": (C.c_void_p, [_f, _u8, C.c_int, _f, _u8, C.c_int]),
+        "lkl_feat_create_grid": (C.c_void_p, [_f, _u8, C.c_int, _f, _u8, C.c_int, C.c_double]),
+        "lkl_feat_grid_info": (C.c_longlong, [C.c_void_p, _d, _i, _d]),
+        "lkl_feat_grid_read": (C.c_int, [C.c_void_p, C.c_longlong, C.c_longlong, _u8]),
+        "lkl_feat_destroy": (None, [C.c_void_p]),
+        "lkl_feat_compute": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p]),
+        "lkl_feat_compute_host": (C.c_int, [C.c_void_p, _d, C.c_int, _u16, _u32]),
+        "lkl_mlp_param_count": (C.c_longlong, []),
+        "lkl_mlp_create": (C.c_void_p, [C.c_uint64]),
+        "lkl_mlp_destroy": (None, [C.c_void_p]),
+        "lkl_mlp_get_params": (C.c_int, [C.c_void_p, _f]),
+        "lkl_mlp_set_params": (C.c_int, [C.c_void_p, _f]),
+        "lkl_mlp_get_grads": (C.c_int, [C.c_void_p, _f]),
+        "lkl_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+        "lkl_mlp_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _f]),
+        "lkl_mlp_adam": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float]),
+        "lkl_mlp_sync": (C.c_int, [C.c_void_p]),
+        "lkl_mlp_forward_host": (C.c_int, [C.c_void_p, _u16, C.c_int, _f]),
+        "lkl_mlp_grad_host": (C.c_int, [C.c_void_p, _u16, _f, C.c_int, _f]),
+        "lkl_gemm_bf16_host": (C.c_int, [_u16, C.c_long, C.c_long, C.c_long, _u16, C.c_long,
+                                         C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, _f]),
         "lkl_rng_f64s": (None, [C.c_uint64, C.c_int, _d]),
         "lkl_quat_slerp": (None, [_d, _d, C.c_double, _d]),
     }

@@ -118,6 +118,19 @@ void orc_gso_run(int family, const orc_mol* rec, const orc_mol* lig, const orc_f
                  double* out_pose, double* out_luc, double* out_score, double* out_vision,
                  int* out_nnb);
 
+/* Surrogate pair-histogram features (lkl_oracle_sur.c): counts n x 6144. */
+void orc_feat_counts(const double* rxyz, const unsigned char* rg, int nr, const double* lxyz,
+                     const unsigned char* lg, int nl, const double* poses, int n, uint32_t* counts,
+                     int nthreads);
+/* Grid-mode features (lkl_oracle_sur.c): box of the node grid, counts of one
+ * node (384 = group x bin), and per-pose features, n x 6144. */
+int64_t orc_feat_grid_dims(const double* rxyz, int nr, double h, double origin[3], int dims[3]);
+void orc_feat_grid_node(const double* rxyz, const unsigned char* rg, int nr, const double origin[3],
+                        double h, int ix, int iy, int iz, uint32_t* out384);
+void orc_feat_grid_counts(const double* rxyz, const unsigned char* rg, int nr, const double* lxyz,
+                          const unsigned char* lg, int nl, const double* poses, int n, double h,
+                          uint32_t* counts, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -86,6 +86,18 @@ def lib():
         L.orc_gso_run.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol), C.POINTER(Field), _d,
                                   C.c_int, C.c_uint64, C.c_int, C.c_int, _d, _d, _d, _d,
                                   C.POINTER(C.c_int)]
+        L.orc_feat_counts.restype = None
+        L.orc_feat_counts.argtypes = [_d, C.POINTER(C.c_ubyte), C.c_int, _d, C.POINTER(C.c_ubyte),
+                                      C.c_int, _d, C.c_int, C.POINTER(C.c_uint32), C.c_int]
+        L.orc_feat_grid_dims.restype = C.c_int64
+        L.orc_feat_grid_dims.argtypes = [_d, C.c_int, C.c_double, _d, C.POINTER(C.c_int)]
+        L.orc_feat_grid_node.restype = None
+        L.orc_feat_grid_node.argtypes = [_d, C.POINTER(C.c_ubyte), C.c_int, _d, C.c_double, C.c_int,
+                                         C.c_int, C.c_int, C.POINTER(C.c_uint32)]
+        L.orc_feat_grid_counts.restype = None
+        L.orc_feat_grid_counts.argtypes = [_d, C.POINTER(C.c_ubyte), C.c_int, _d, C.POINTER(C.c_ubyte),
+                                           C.c_int, _d, C.c_int, C.c_double, C.POINTER(C.c_uint32),
+                                           C.c_int]
         L.orc_nearcells_pair_count.restype = C.c_int64
         L.orc_nearcells_pair_count.argtypes = [_d, C.c_int, C.c_double]
         _lib = L
@@ -246,3 +258,49 @@ def crec_build(xyz):
     ca = np.zeros(max(1, xyz.shape[0]), np.int32)
     L.orc_crec_build(_p(xyz), xyz.shape[0], nc, co, _p(cs, C.c_int), _p(ca, C.c_int))
     return cs, ca[:xyz.shape[0]], tuple(nc), tuple(co)
+
+
+def feat_counts(rec_xyz, rec_group, lig_xyz, lig_group, poses, nthreads=8):
+    """Surrogate pair-histogram counts (lkl_oracle_sur.c), n x 6144 u32."""
+    rx = np.ascontiguousarray(rec_xyz, np.float64)
+    lx = np.ascontiguousarray(lig_xyz, np.float64)
+    rg = np.ascontiguousarray(rec_group, np.uint8)
+    lg = np.ascontiguousarray(lig_group, np.uint8)
+    P = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
+    out = np.zeros((P.shape[0], 6144), np.uint32)
+    lib().orc_feat_counts(_p(rx), _p(rg, C.c_ubyte), rx.shape[0], _p(lx), _p(lg, C.c_ubyte),
+                          lx.shape[0], _p(P), P.shape[0], _p(out, C.c_uint32), int(nthreads))
+    return out
+
+
+def feat_grid_dims(rec_xyz, h):
+    rx = np.ascontiguousarray(rec_xyz, np.float64)
+    o = np.zeros(3)
+    d = np.zeros(3, np.int32)
+    n = lib().orc_feat_grid_dims(_p(rx), rx.shape[0], float(h), _p(o), _p(d, C.c_int))
+    return o, tuple(int(v) for v in d), int(n)
+
+
+def feat_grid_node(rec_xyz, rec_group, h, ix, iy, iz):
+    """Counts (16 x 24) of one node of the grid-mode table."""
+    rx = np.ascontiguousarray(rec_xyz, np.float64)
+    rg = np.ascontiguousarray(rec_group, np.uint8)
+    o, _, _ = feat_grid_dims(rx, h)
+    out = np.zeros(384, np.uint32)
+    lib().orc_feat_grid_node(_p(rx), _p(rg, C.c_ubyte), rx.shape[0], _p(o), float(h), int(ix), int(iy),
+                             int(iz), _p(out, C.c_uint32))
+    return out
+
+
+def feat_grid_counts(rec_xyz, rec_group, lig_xyz, lig_group, poses, h, nthreads=8):
+    """Grid-mode features (nearest-node receptor counts), n x 6144 u32."""
+    rx = np.ascontiguousarray(rec_xyz, np.float64)
+    lx = np.ascontiguousarray(lig_xyz, np.float64)
+    rg = np.ascontiguousarray(rec_group, np.uint8)
+    lg = np.ascontiguousarray(lig_group, np.uint8)
+    P = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
+    out = np.zeros((P.shape[0], 6144), np.uint32)
+    lib().orc_feat_grid_counts(_p(rx), _p(rg, C.c_ubyte), rx.shape[0], _p(lx), _p(lg, C.c_ubyte),
+                               lx.shape[0], _p(P), P.shape[0], float(h), _p(out, C.c_uint32),
+                               int(nthreads))
+    return out
