@@ -67,8 +67,20 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
-                    help="c2: 12.6k-atom metric workload; c4: 64-complex batch")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
+                    help="c2: 12.6k-atom metric workload; c4: 64-complex batch; "
+                         "c5: surrogate-screened GSO vs plain exact GSO")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="c2: skip the surrogate-screened GSO comparison")
+    ap.add_argument("--sur-train", type=int, default=2_000_000, help="c5: training poses")
+    ap.add_argument("--sur-epochs", type=int, default=6)
+    ap.add_argument("--sur-swarms", type=int, default=40)
+    ap.add_argument("--sur-glow", type=int, default=50)
+    ap.add_argument("--sur-plain-steps", type=int, default=100)
+    ap.add_argument("--sur-max-steps", type=int, default=100)
+    ap.add_argument("--sur-grid", type=float, default=0.5)
+    ap.add_argument("--sur-trans", type=float, default=2.0)
+    ap.add_argument("--sur-rot", type=float, default=0.1)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="run only a plain 512 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -99,6 +111,9 @@ def main():
     dev = torch.device("cuda:0")
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        print(json.dumps(surrogate_experiment(args)))
+        return
 
     log("generating 12,625-atom receptor / 1,500-atom ligand")
     cx = synth.make_config("c2")
@@ -201,8 +216,110 @@ def main():
                      "n_near_per_pose": n_near, "n_far_per_pose": n_far},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(out))
     ctx.close()
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_experiment(args)
+    print(json.dumps(out))
+
+
+def surrogate_experiment(args):
+    """Config-5 comparison on the 1,500-atom / 800-atom complex: train the
+    surrogate on the GPU from exact dna scores, then run plain exact GSO and the
+    surrogate-screened GSO (and the same screen with random picks) from the
+    same start; report rank correlation, top-10 recovery and wall times."""
+    import numpy as np
+    import torch
+    import lklight
+    from lklight import screen, synth
+    from lklight import surrogate as S
+
+    method = "dna"
+    cx = synth.make_config("c1")
+    cx.build_field()
+    ctx = lklight.Context(cx)
+    feats = S.Features(cx, grid=args.sur_grid)
+    ns, ng = args.sur_swarms, args.sur_glow
+    sizes = [ng] * ns
+    pos = synth.make_swarm_positions(ns, ng, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
+    # training poses: candidates along a 100-step plain GSO trajectory from
+    # other starts (every k-th step, ~80 %) + random poses in the 15 A shell
+    t0 = time.perf_counter()
+    n_tr = args.sur_train
+    per_step = ns * ng * 32
+    every = max(1, (100 * per_step) // max(1, (n_tr * 4) // 5))
+    pos_tr = synth.make_swarm_positions(ns, ng, cx.meta["rec_axes"], 4242, lig_radius=cx.meta["lig_axes"][0])
+    traj = screen.collect_poses(ctx, method, pos_tr, 4242, sizes, 100, 32, args.sur_trans, args.sur_rot,
+                                every=every)[:n_tr]
+    rnd = synth.make_poses(max(0, n_tr - traj.shape[0]), cx.meta["rec_axes"], 77)
+    P = np.concatenate([traj, rnd], 0)
+    scores = np.concatenate([screen.exact_scores(ctx, method, P[i:i + 65536])
+                             for i in range(0, P.shape[0], 65536)])
+    X = screen.features_of(feats, P)
+    tmap = screen.Target(scores)
+    T = torch.from_numpy(tmap(scores)).cuda()
+    t_data = time.perf_counter() - t0
+    log("surrogate data: %d poses (%d trajectory candidates) in %.1f s" % (P.shape[0], traj.shape[0], t_data))
+    net = S.MLP(2024)
+    t0 = time.perf_counter()
+    hist = screen.train(net, X[:P.shape[0]], T, epochs=args.sur_epochs, log=log)
+    t_train = time.perf_counter() - t0
+    del X
+    # held-out: candidates from the benchmark start (different swarms)
+    # held-out: candidates at steps 0, 20 and 40 of a plain GSO from the benchmark start
+    hold = screen.collect_poses(ctx, method, pos, 99, sizes, 41, 32, args.sur_trans, args.sur_rot, every=20)
+    hs = screen.exact_scores(ctx, method, hold)
+    hp = screen.predict(net, screen.features_of(feats, hold), hold.shape[0])
+    rho = screen.spearman(hp, hs)
+    rho_pg = float(np.mean([screen.spearman(hp[i:i + 32], hs[i:i + 32]) for i in range(0, len(hs), 32)]))
+    log("held-out spearman %.3f (within-glowworm mean %.3f)" % (rho, rho_pg))
+    # plain exact GSO (the judge)
+    plain_sc, plain_wall, _ = screen.run_plain(ctx, method, pos, 42, sizes, args.sur_plain_steps)
+    ref10 = screen.top_k(plain_sc)
+    log("plain GSO %d steps: %.3f s, top-10 %s" % (args.sur_plain_steps, plain_wall, np.round(ref10, 2)))
+
+    def screened(picker):
+        sg = screen.ScreenedGSO(ctx, feats, net, method, pos, 42, sizes, trans_radius=args.sur_trans,
+                                rot_sigma=args.sur_rot, picker=picker)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reached, steps_done = None, 0
+        for k in range(args.sur_max_steps):
+            sg.step()
+            steps_done = k + 1
+            top = screen.top_k(sg.state()["scoring"])
+            if screen.recovered(top, ref10):
+                reached = time.perf_counter() - t0
+                break
+        wall = time.perf_counter() - t0
+        top = screen.top_k(sg.state()["scoring"])
+        res = {"steps": steps_done, "wall_s": wall, "reached_s": reached, "top10": top.tolist(),
+               "exact_evals": sg.exact_evals, "surrogate_evals": sg.surrogate_evals}
+        sg.close()
+        return res
+
+    sur = screened("surrogate")
+    rnd_res = screened("random")
+    log("screened: %s" % {k: v for k, v in sur.items() if k != "top10"})
+    log("random picks: %s" % {k: v for k, v in rnd_res.items() if k != "top10"})
+    # None when the screened run never reached the plain run's top-10 (+-0.5 %)
+    speedup = plain_wall / sur["reached_s"] if sur["reached_s"] else None
+    sur["ms_per_step"] = sur["wall_s"] / sur["steps"] * 1e3
+    rnd_res["ms_per_step"] = rnd_res["wall_s"] / rnd_res["steps"] * 1e3
+    out = {
+        "workload": "c5: 1,500-atom receptor x 800-atom ligand, %d swarms x %d glowworms, 32 candidates "
+                    "per glowworm, top 2 exact-scored, grid features h=%.2g A" % (ns, ng, args.sur_grid),
+        "speedup_vs_plain": speedup,
+        "plain": {"steps": args.sur_plain_steps, "wall_s": plain_wall, "top10": ref10.tolist(),
+                  "ms_per_step": plain_wall / args.sur_plain_steps * 1e3},
+        "screened": sur, "random_picks": rnd_res,
+        "spearman_heldout": rho, "spearman_within_glowworm": rho_pg,
+        "train": {"poses": int(P.shape[0]), "epochs": args.sur_epochs, "data_s": t_data,
+                  "train_s": t_train, "loss": hist},
+    }
+    feats.close()
+    net.close()
+    ctx.close()
+    return out
 
 
 def bench_c4(args):

@@ -254,6 +254,17 @@ int lkl_gso_state(const lkl_gso* gso, double* pose7, double* luciferin, double* 
 /* Replace the poses of the glowworms selected by mask (NULL = all) and mark
  * them moved (no reference counterpart: used by surrogate-screened moves). */
 int lkl_gso_set_poses(lkl_gso* gso, const double* pose7, const unsigned char* mask);
+/* Surrogate-screened movement (no reference counterpart), first half: plan
+ * every swarm's GSO move exactly as lkl_gso_movement_phase would (same draws,
+ * same vision update) and write the planned poses (rows w, x, y, z, tx, ty, tz;
+ * the current pose where a glowworm has no brighter neighbour) into moves.
+ * Nothing moves until lkl_gso_commit.  0 / -1. */
+int lkl_gso_plan(lkl_gso* gso, double* moves);
+/* Second half: glowworms with accept[k] != 0 take pose7[k] and its exact score
+ * scores[k]; no glowworm is marked moved (the next update_luciferin scores
+ * nothing).  0 / -1. */
+int lkl_gso_commit(lkl_gso* gso, const double* pose7, const double* scores,
+                   const unsigned char* accept);
 /* GPU launches, fallback calls and poses scored on the GPU so far. */
 int lkl_gso_stats(const lkl_gso* gso, long long* gpu_launches, long long* fallback_calls,
                   long long* scored);
@@ -333,6 +344,21 @@ int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* t, int B
 int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, long sak,
                        const unsigned short* B, long b_len, long sbk, long sbn, int M, int N,
                        int K, float* C);
+
+/* Screening step (no reference counterpart).  Candidates: n base poses
+ * (device, rows w, x, y, z, tx, ty, tz) -> n * n_cand poses, glowworm-major;
+ * candidate 0 is the base pose, candidate c > 0 adds trans_radius * U(-1, 1) to
+ * each translation component and rot_sigma * U(-1, 1) to each quaternion
+ * component (then renormalised).  Draws are counter-based:
+ * u = (splitmix64(seed ^ splitmix64(8 i + d)) >> 11) * 2^-53 for candidate i,
+ * draw d (0-3 quaternion, 4-6 translation).  0 / -1. */
+int lkl_screen_candidates(const double* d_base, int n, int n_cand, double trans_radius,
+                          double rot_sigma, unsigned long long seed, double* d_cand, void* stream);
+/* Per glowworm, the k (1..8) candidates with the highest prediction (ties:
+ * lower index first): indices (n x k) and poses (n x k x 7), device buffers on
+ * `stream`.  0 / -1. */
+int lkl_screen_select(const float* d_pred, const double* d_cand, int n, int n_cand, int k,
+                      int* d_pick, double* d_sel, void* stream);
 
 #ifdef __cplusplus
 }

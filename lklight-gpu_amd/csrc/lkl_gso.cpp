@@ -121,13 +121,22 @@ constexpr double kRho = 0.5, kGamma = 0.4, kBeta = 0.08, kMaxVision = 5.0;
 constexpr int kMaxNeighbors = 5;
 constexpr double kTransStep = 0.5, kRotStep = 0.5;  // constants.rs:5-8
 
+// A glowworm's planned move (the current pose when it has no brighter neighbour).
+struct Move {
+    std::array<double, 3> t;
+    Quat q;
+    bool moved;
+};
+
 struct Swarm {
     std::vector<Glowworm> g;
     StdRng rng;
     explicit Swarm(uint64_t seed) : rng(seed) {}
 
-    // swarm.rs:105-174 with glowworm.rs:84-166
-    void movement_phase() {
+    // swarm.rs:105-174 with glowworm.rs:84-166, without applying the moves:
+    // neighbours and probabilities from the step's snapshot, one draw per
+    // glowworm in id order, the chosen neighbour's move, the vision update.
+    std::vector<Move> plan_moves() {
         const size_t n = g.size();
         std::vector<std::array<double, 3>> pos(n);
         std::vector<Quat> rot(n);
@@ -152,6 +161,7 @@ struct Swarm {
         }
         std::vector<double> rnd(n);
         for (double& r : rnd) r = rng.next_f64();  // serial draws, id order (swarm.rs:165)
+        std::vector<Move> mv(n);
         for (size_t i = 0; i < n; ++i) {
             Glowworm& a = g[i];
             size_t nid = i;
@@ -161,21 +171,33 @@ struct Swarm {
                 while (sum < rnd[i] && m < a.nb.size()) sum += a.prob[m++];
                 nid = (size_t)a.nb[m > 0 ? m - 1 : 0];
             }
-            a.moved = nid != i;
-            if (a.moved) {  // glowworm.rs:128-150
+            mv[i] = {a.t, a.q, nid != i};
+            if (mv[i].moved) {  // glowworm.rs:128-150
                 const double dx = pos[nid][0] - a.t[0], dy = pos[nid][1] - a.t[1],
                              dz = pos[nid][2] - a.t[2];
                 const double norm = std::sqrt(dx * dx + dy * dy + dz * dz);
                 const double coef = kTransStep / norm;
-                a.t[0] += dx * coef;
-                a.t[1] += dy * coef;
-                a.t[2] += dz * coef;
-                a.q = a.q.slerp(rot[nid], kRotStep);
-                a.q.normalize();
+                mv[i].t[0] += dx * coef;
+                mv[i].t[1] += dy * coef;
+                mv[i].t[2] += dz * coef;
+                mv[i].q = a.q.slerp(rot[nid], kRotStep);
+                mv[i].q.normalize();
             }
             // glowworm.rs:91-96
             a.vision = std::fmin(kMaxVision,
                                  std::fmax(0.0, a.vision + kBeta * (double)(kMaxNeighbors - (int)a.nb.size())));
+        }
+        return mv;
+    }
+
+    void movement_phase() {
+        const std::vector<Move> mv = plan_moves();
+        for (size_t i = 0; i < g.size(); ++i) {
+            g[i].moved = mv[i].moved;
+            if (mv[i].moved) {
+                g[i].t = mv[i].t;
+                g[i].q = mv[i].q;
+            }
         }
     }
 };
@@ -341,6 +363,45 @@ LKL_API int lkl_gso_set_poses(lkl_gso* G, const double* pose7, const unsigned ch
                 w.t = {p[4], p[5], p[6]};
                 w.moved = true;
             }
+            ++k;
+        }
+    return 0;
+}
+
+// Screened movement, first half: plan every swarm's GSO moves with the same
+// draws and vision update as lkl_gso_movement_phase, and write the planned
+// poses (rows w, x, y, z, tx, ty, tz; the current pose for a glowworm with no
+// brighter neighbour) without moving anything.
+LKL_API int lkl_gso_plan(lkl_gso* G, double* moves) {
+    if (!G || !moves) return -1;
+    size_t k = 0;
+    for (Swarm& sw : G->swarms) {
+        const std::vector<Move> mv = sw.plan_moves();
+        for (size_t i = 0; i < sw.g.size(); ++i, ++k) {
+            sw.g[i].moved = false;
+            double* p = moves + 7 * k;
+            p[0] = mv[i].q.w; p[1] = mv[i].q.x; p[2] = mv[i].q.y; p[3] = mv[i].q.z;
+            p[4] = mv[i].t[0]; p[5] = mv[i].t[1]; p[6] = mv[i].t[2];
+        }
+    }
+    return 0;
+}
+
+// Screened movement, second half: glowworms with accept[k] set take pose7[k]
+// and its (exact) score; the others keep pose and score.  Nothing is marked
+// moved, so the next update_luciferin scores nothing.
+LKL_API int lkl_gso_commit(lkl_gso* G, const double* pose7, const double* scores, const unsigned char* accept) {
+    if (!G || !pose7 || !scores || !accept) return -1;
+    size_t k = 0;
+    for (Swarm& sw : G->swarms)
+        for (Glowworm& w : sw.g) {
+            if (accept[k]) {
+                const double* p = pose7 + 7 * k;
+                w.q = {p[0], p[1], p[2], p[3]};
+                w.t = {p[4], p[5], p[6]};
+                w.scoring = scores[k];
+            }
+            w.moved = false;
             ++k;
         }
     return 0;

@@ -267,6 +267,72 @@ feat_grid_kernel(GridDev G, const float4* __restrict__ lig, int nl, const unsign
     }
 }
 
+// ── screening-step kernels ────────────────────────────────────────────────────
+// Counter-based draws: u = (splitmix64(key) >> 11) * 2^-53 with key derived
+// from (seed, candidate index, draw index), so a candidate set depends only on
+// its inputs (reproducible, no generator state on the device).
+__host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double draw_u01(unsigned long long seed, long long i, int d) {
+    return (double)(mix64(seed ^ mix64((unsigned long long)i * 8ull + (unsigned long long)d)) >> 11) *
+           (1.0 / 9007199254740992.0);
+}
+
+// cand[i], i = g * C + c: c = 0 copies base[g]; c > 0 adds trans_radius * U(-1, 1)
+// to each translation component and rot_sigma * U(-1, 1) to each quaternion
+// component, then renormalises the quaternion.
+__global__ void cand_kernel(const double* __restrict__ base, long long total, int C, double tr, double rs,
+                            unsigned long long seed, double* __restrict__ cand) {
+#pragma clang fp contract(off)
+    const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const double* b = base + (i / C) * 7;
+    double* o = cand + i * 7;
+    if (i % C == 0) {
+        for (int k = 0; k < 7; ++k) o[k] = b[k];
+        return;
+    }
+    double q[4];
+    for (int k = 0; k < 4; ++k) q[k] = b[k] + rs * (2.0 * draw_u01(seed, i, k) - 1.0);
+    const double nrm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) o[k] = q[k] / nrm;
+    for (int k = 0; k < 3; ++k) o[4 + k] = b[4 + k] + tr * (2.0 * draw_u01(seed, i, 4 + k) - 1.0);
+}
+
+// Per glowworm g: the k candidates with the highest prediction (ties: lower
+// index first), their indices in pick[g * k ..] and poses in sel.
+constexpr int SEL_MAX = 8;
+__global__ void select_kernel(const float* __restrict__ pred, const double* __restrict__ cand, int n, int C,
+                              int k, int* __restrict__ pick, double* __restrict__ sel) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    int best[SEL_MAX];
+    float val[SEL_MAX];
+    int m = 0;
+    for (int c = 0; c < C; ++c) {
+        const float v = pred[(size_t)g * C + c];
+        int pos = m;
+        while (pos > 0 && !(val[pos - 1] >= v)) --pos;  // keep earlier index on ties
+        if (pos >= k) continue;
+        if (m < k) ++m;
+        for (int q = m - 1; q > pos; --q) { val[q] = val[q - 1]; best[q] = best[q - 1]; }
+        val[pos] = v;
+        best[pos] = c;
+    }
+    for (int r = 0; r < k; ++r) {
+        const int c = best[r < m ? r : m - 1];
+        pick[(size_t)g * k + r] = c;
+        const double* src = cand + ((size_t)g * C + c) * 7;
+        double* dst = sel + ((size_t)g * k + r) * 7;
+        for (int q = 0; q < 7; ++q) dst[q] = src[q];
+    }
+}
+
 // ── GEMM: C[M x N] = sum_k A(m, k) * B(k, n), bf16 in, fp32 accumulate ────────
 // A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn] (one of each
 // pair of strides is 1).  M, N multiples of 128, K of 32.  Epilogues:
@@ -958,4 +1024,42 @@ LKL_API int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, lo
     if (dB) (void)hipFree(dB);
     if (dC) (void)hipFree(dC);
     return rc;
+}
+
+// ── screening step C ABI ──────────────────────────────────────────────────────
+// n base poses (device, rows w, x, y, z, tx, ty, tz) -> n * n_cand candidates
+// (glowworm-major) on `stream`.  0 / -1.
+LKL_API int lkl_screen_candidates(const double* d_base, int n, int n_cand, double trans_radius,
+                                  double rot_sigma, unsigned long long seed, double* d_cand, void* stream) {
+    if (!d_base || !d_cand || n < 0 || n_cand <= 0 || trans_radius < 0.0 || rot_sigma < 0.0) {
+        serr("lkl_screen_candidates: bad arguments");
+        return -1;
+    }
+    const long long total = (long long)n * n_cand;
+    if (total == 0) return 0;
+    hipLaunchKernelGGL(cand_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_base, total, n_cand, trans_radius, rot_sigma, seed, d_cand);
+    if (hipGetLastError() != hipSuccess) {
+        serr("lkl_screen_candidates: launch failed");
+        return -1;
+    }
+    return 0;
+}
+
+// Per glowworm, the k (1..8) best of its n_cand predictions: indices into
+// d_pick (n x k) and poses into d_sel (n x k x 7), on `stream`.  0 / -1.
+LKL_API int lkl_screen_select(const float* d_pred, const double* d_cand, int n, int n_cand, int k,
+                              int* d_pick, double* d_sel, void* stream) {
+    if (!d_pred || !d_cand || !d_pick || !d_sel || n < 0 || n_cand <= 0 || k < 1 || k > SEL_MAX || k > n_cand) {
+        serr("lkl_screen_select: bad arguments (need 1 <= k <= min(8, n_cand))");
+        return -1;
+    }
+    if (n == 0) return 0;
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       d_pred, d_cand, n, n_cand, k, d_pick, d_sel);
+    if (hipGetLastError() != hipSuccess) {
+        serr("lkl_screen_select: launch failed");
+        return -1;
+    }
+    return 0;
 }

@@ -56,6 +56,14 @@ def lib():
         raise LibraryMissing(
             f"{LIB_PATH} is missing: build it with `python lklight-gpu_amd/build.py` "
             "(there is no CPU substitute for the HIP path)")
+    # PyTorch (device memory and streams for callers) ships its own HIP runtime
+    # with the same soname as /opt/rocm's.  Loading torch first lets this
+    # library bind to that runtime, so the process holds ONE HIP runtime; the
+    # other order leaves two, and the second cannot open the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     sig = {
         "lkl_device_available": (C.c_int, []),
@@ -110,6 +118,8 @@ def lib():
         "lkl_gso_stats": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
                                     C.POINTER(C.c_longlong)]),
         "lkl_gso_save": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_char_p]),
+        "lkl_gso_plan": (C.c_int, [C.c_void_p, _d]),
+        "lkl_gso_commit": (C.c_int, [C.c_void_p, _d, _d, _u8]),
         "lkl_feat_create": (C.c_void_p, [_f, _u8, C.c_int, _f, _u8, C.c_int]),
         "lkl_feat_create_grid": (C.c_void_p, [_f, _u8, C.c_int, _f, _u8, C.c_int, C.c_double]),
         "lkl_feat_grid_info": (C.c_longlong, [C.c_void_p, _d, _i, _d]),
@@ -132,6 +142,10 @@ def lib():
         "lkl_mlp_grad_host": (C.c_int, [C.c_void_p, _u16, _f, C.c_int, _f]),
         "lkl_gemm_bf16_host": (C.c_int, [_u16, C.c_long, C.c_long, C.c_long, _u16, C.c_long,
                                          C.c_long, C.c_long, C.c_int, C.c_int, C.c_int, _f]),
+        "lkl_screen_candidates": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                            C.c_uint64, C.c_void_p, C.c_void_p]),
+        "lkl_screen_select": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
         "lkl_rng_f64s": (None, [C.c_uint64, C.c_int, _d]),
         "lkl_quat_slerp": (None, [_d, _d, C.c_double, _d]),
     }

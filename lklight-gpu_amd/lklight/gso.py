@@ -76,6 +76,20 @@ class GSO:
         _capi.check(lib().lkl_gso_set_poses(self.h, _capi.ptr(p, C.c_double),
                                             _capi.ptr(m, C.c_ubyte)), "lkl_gso_set_poses")
 
+    def plan(self):
+        """Plan the step's GSO moves (same draws as movement_phase) without
+        applying them; returns the planned poses (n, 7) as (w, x, y, z, tx, ty, tz)."""
+        out = np.zeros((self.n, 7))
+        _capi.check(lib().lkl_gso_plan(self.h, _capi.ptr(out, C.c_double)), "lkl_gso_plan")
+        return out
+
+    def commit(self, pose7, scores, accept):
+        p = _capi.as_f64(pose7).reshape(-1, 7)
+        s = _capi.as_f64(scores)
+        a = _capi.as_u8(accept)
+        _capi.check(lib().lkl_gso_commit(self.h, _capi.ptr(p, C.c_double), _capi.ptr(s, C.c_double),
+                                         _capi.ptr(a, C.c_ubyte)), "lkl_gso_commit")
+
     def stats(self):
         a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
         lib().lkl_gso_stats(self.h, C.byref(a), C.byref(b), C.byref(c))

@@ -44,9 +44,14 @@ class Mol:
 
     @property
     def group(self):
-        """Type group (0..15) of the surrogate's pair histograms."""
-        t = self.typ if self.typ is not None else np.zeros(self.n, np.int64)
-        return (np.asarray(t) % 16).astype(np.uint8)
+        """Type group (0..15) of the surrogate's pair histograms: 4 size
+        classes of the 20-entry type table (types 0-4, 5-9, 10-14, 15-19) x 4
+        charge classes (q < -0.2, -0.2 <= q < 0, 0 <= q < 0.2, q >= 0.2).  The
+        synthetic charges are drawn per atom, not per type, so the charge class
+        is what lets the histograms see electrostatics."""
+        t = np.asarray(self.typ if self.typ is not None else np.zeros(self.n, np.int64))
+        qc = np.searchsorted(np.array([-0.2, 0.0, 0.2]), np.asarray(self.ele), side="right")
+        return ((t * 4) // 20 * 4 + qc).astype(np.uint8)
 
     @property
     def n(self):

@@ -62,8 +62,10 @@ def test_features_exact_thresholds(gpu):
     cx = synth.make_config("tiny")
     cx.rec.xyz = np.zeros((1, 3))
     cx.rec.typ = np.zeros(1, np.int32)
+    cx.rec.ele = np.zeros(1)
     cx.lig.xyz = np.stack([xs, np.zeros_like(xs), np.zeros_like(xs)], 1)
     cx.lig.typ = np.zeros(len(xs), np.int32)
+    cx.lig.ele = np.zeros(len(xs))
     f = S.Features(cx)
     got = f.counts(np.array([[1, 0, 0, 0, 0, 0, 0.0]]))[0]
     exp = O.feat_counts(cx.rec.xyz, cx.rec.group, cx.lig.xyz, cx.lig.group,
@@ -92,6 +94,9 @@ def test_features_large_counts_and_batching(gpu):
     cx = synth.make_config("c1")
     cx.rec.typ = np.zeros(cx.rec.n, np.int32)  # one group: large counts per bin
     cx.lig.typ = np.zeros(cx.lig.n, np.int32)
+    cx.rec.ele = np.zeros(cx.rec.n)
+    cx.lig.ele = np.zeros(cx.lig.n)
+    assert len(set(cx.rec.group)) == 1 and len(set(cx.lig.group)) == 1
     poses = synth.make_poses(6, cx.meta["rec_axes"], 11, shell=0.0)
     f = S.Features(cx)
     cnt = f.counts(poses)

@@ -149,3 +149,40 @@ def test_gso_dna_multi_swarm_one_launch_per_step(gpu):
         one.close()
     g.close()
     ctx.close()
+
+
+def test_plan_then_set_poses_equals_movement_phase(small_vdw):
+    """lkl_gso_plan consumes the same draws and writes the same moves as
+    movement_phase: following the plan reproduces the plain trajectory."""
+    cx, orc, pos = small_vdw
+
+    def cpu_scores(poses):
+        return -orc.batch("vdw", poses)["v"]
+
+    a = G.GSO(None, "vdw", pos, seed=11, fallback=cpu_scores)
+    b = G.GSO(None, "vdw", pos, seed=11, fallback=cpu_scores)
+    for _ in range(6):
+        a.update_luciferin()
+        a.movement_phase()
+        b.update_luciferin()
+        plan = b.plan()
+        moved = a.state()["moved"]
+        b.set_poses(plan, mask=moved)
+        assert np.array_equal(a.state()["pose"], b.state()["pose"])
+        assert np.array_equal(a.state()["vision"], b.state()["vision"])
+    a.close()
+    b.close()
+
+
+def test_commit_sets_pose_and_score(small_vdw):
+    cx, orc, pos = small_vdw
+    g = G.GSO(None, "vdw", pos[:4], seed=1, fallback=lambda p: np.zeros(p.shape[0]))
+    g.update_luciferin()
+    plan = g.plan()
+    acc = np.array([1, 0, 1, 0], np.uint8)
+    g.commit(plan, np.array([1.5, 2.5, 3.5, 4.5]), acc)
+    st = g.state()
+    assert np.array_equal(st["pose"][[0, 2]], plan[[0, 2]])
+    assert list(st["scoring"]) == [1.5, 0.0, 3.5, 0.0]
+    assert not st["moved"].any()
+    g.close()
