@@ -75,6 +75,20 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 // the 10.025 A ball, accumulate the pair terms of full_score.cu:296-315 /
 // dna.rs:951-969 and (with desolvation) the per-atom minima of
 // cpydock.rs:620-627.
+#ifndef LKL_NU
+#define LKL_NU 4
+#endif
+#ifndef LKL_PRE_RPAR
+#define LKL_PRE_RPAR 0
+#endif
+#ifndef SQRT_W
+#define SQRT_W(x) __builtin_amdgcn_sqrtf(x)
+#endif
+#ifndef SQRT_D
+#define SQRT_D(x) __builtin_amdgcn_sqrtf(x)
+#endif
+constexpr int NU = LKL_NU;
+
 template <bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, double y64, double z64,
                                           float qj, float svj, float rj, bool hj, bool lf,
@@ -101,51 +115,73 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
             const float gy = fmaxf(0.0f, fmaxf((float)iy - fy, fy - (float)(iy + 1)));
             const float g2 = gz2 + gy * gy;
             if (g2 > rw2) continue;
-            const float b = sqrtf(rw2 - g2);
+            const float b = SQRT_W(rw2 - g2);
             const int xlo = max(0, (int)floorf(fx - b));
             const int xhi = min(C.ncx - 1, (int)floorf(fx + b));
             if (xlo > xhi) continue;
             const int row = (iz * C.ncy + iy) * C.ncx;
-            int p = C.cstart[row + xlo];
+            const int pb = C.cstart[row + xlo];
             const int pe = C.cstart[row + xhi + 1];
-            if (p >= pe) continue;
-            float4 rn = C.rxyzq[p];
-            for (; p < pe; ++p) {
-                const float4 r = rn;  // next record is requested before this one is used
-                if (p + 1 < pe) rn = C.rxyzq[p + 1];
-                const float dx = (xh - r.x) + xl;
-                const float dy = (yh - r.y) + yl;
-                const float dz = (zh - r.z) + zl;
-                const float d2 = dx * dx + dy * dy + dz * dz;
-                if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
-                    if (C.rflag[p] == 0) {
-                        const double d2e = exact_d2(x64, y64, z64, r);
-                        minL = fmin(minL, d2e);
-                        atomicMin(&minR[p], solv_key(d2e));
-                    }
+            // NU candidates per step: their records are requested together
+            // (indices clamped into the row, so every load is in bounds) and
+            // the distance tests then run on data already in flight.
+            for (int p0 = pb; p0 < pe; p0 += NU) {
+                float4 rq[NU];
+#if LKL_PRE_RPAR
+                float4 rpq[NU];
+#endif
+#pragma unroll
+                for (int k = 0; k < NU; ++k) {
+                    const int pk = min(p0 + k, pe - 1);
+                    rq[k] = C.rxyzq[pk];
+#if LKL_PRE_RPAR
+                    rpq[k] = C.rpar[pk];
+#endif
                 }
-                if (d2 <= NEAR_HI) {
-                    if (d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2) continue;
-                    const float4 rp = C.rpar[p];
-                    const float inv = __builtin_amdgcn_rcpf(d2);
-                    if (ELEC) {
-                        const float ae = qjs * r.w * inv;
-                        e += ae > ES_CAP_SF ? ES_CAP_I
-                                            : (ae < -ES_CAP_SF ? -ES_CAP_I : (long long)__float2int_rn(ae));
+#pragma unroll
+                for (int k = 0; k < NU; ++k) {
+                    const int p = p0 + k;
+                    if (p >= pe) break;
+                    const float4 r = rq[k];
+                    const float dx = (xh - r.x) + xl;
+                    const float dy = (yh - r.y) + yl;
+                    const float dz = (zh - r.z) + zl;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
+                        if (C.rflag[p] == 0) {
+                            const double d2e = exact_d2(x64, y64, z64, r);
+                            minL = fmin(minL, d2e);
+                            atomicMin(&minR[p], solv_key(d2e));
+                        }
                     }
-                    const float svs = svjs * rp.x;
-                    const float rr = rj + rp.y;
-                    const float rr2 = rr * rr;
-                    const float i2 = inv * inv;
-                    const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
-                    float vp = svs * (p6 * p6 - 2.0f * p6);
-                    vp = fminf(vp, LJ_CAP * V_SCALE_F);
-                    if (CLASH && hj && rp.z != 0.0f) {
-                        const float d = sqrtf(d2);
-                        const float dmin = CP_F * rr;
-                        if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+                    if (d2 <= NEAR_HI && !(d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2)) {
+#if LKL_PRE_RPAR
+                        const float4 rp = rpq[k];
+#else
+                        const float4 rp = C.rpar[p];
+#endif
+                        const float inv = __builtin_amdgcn_rcpf(d2);
+                        if (ELEC) {  // clamp to +-ES_CAP (selects, no branches)
+                            const float ae = qjs * r.w * inv;
+                            int te = __float2int_rn(ae);
+                            te = ae > ES_CAP_SF ? (int)ES_CAP_I : te;
+                            te = ae < -ES_CAP_SF ? -(int)ES_CAP_I : te;
+                            e += te;
+                        }
+                        const float svs = svjs * rp.x;
+                        const float rr = rj + rp.y;
+                        const float rr2 = rr * rr;
+                        const float i2 = inv * inv;
+                        const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
+                        float vp = svs * (p6 * p6 - 2.0f * p6);
+                        vp = fminf(vp, LJ_CAP * V_SCALE_F);
+                        if (CLASH && hj && rp.z != 0.0f) {
+                            const float d = SQRT_D(d2);
+                            const float dmin = CP_F * rr;
+                            if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+                        }
+                        v += (long long)__float2int_rn(vp);
                     }
-                    v += (long long)__float2int_rn(vp);
                 }
             }
         }
@@ -159,8 +195,11 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
 // the last tile of a pose (counter cnt[k]) writes the result and re-zeroes acc
 // and cnt for the next launch.  Desolvation needs the whole ligand in one tile.
 // PERATOM also writes each ligand atom's (E, V) (sorted order) to pa_e/pa_v.
+#ifndef LKL_SCORE_ATTR
+#define LKL_SCORE_ATTR
+#endif
 template <bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(512) LKL_SCORE_ATTR
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
              float* __restrict__ pa_v, long long* __restrict__ acc, unsigned* __restrict__ cnt,
