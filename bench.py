@@ -39,6 +39,23 @@ def pose_bytes(nl, n_far, n_near, desolv=False, nr=0):
     return b
 
 
+TRAFFIC_FILE = "profiles/score_kernel_c2_dna_counters.json"
+
+
+def measured_traffic(workload, family, n):
+    """HBM bytes per launch of the scoring kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), for
+    the workload they were taken on; None otherwise."""
+    try:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != "%s-%s" % (workload, family) or d.get("poses_per_launch") != n:
+        return None
+    return d["traffic_bytes_per_launch"]
+
+
 def cpu_baseline(cx, poses, method, threads, budget_s=15.0):
     """The oracle (C restatement of the reference CPU grid path, f64) on the
     host cores, parallel over poses like the reference's rayon loop."""
@@ -67,9 +84,9 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2: 12.6k-atom metric workload; c4: 64-complex batch; "
-                         "c5: surrogate-screened GSO vs plain exact GSO")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: 12.6k-atom metric workload; c3: 8k x 3k cpydock, 400 poses/step; "
+                         "c4: 64-complex batch; c5: surrogate-screened GSO vs plain exact GSO")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="c2: skip the surrogate-screened GSO comparison")
     ap.add_argument("--sur-train", type=int, default=2_000_000, help="c5: training poses")
@@ -115,8 +132,13 @@ def main():
         print(json.dumps(surrogate_experiment(args)))
         return
 
-    log("generating 12,625-atom receptor / 1,500-atom ligand")
-    cx = synth.make_config("c2")
+    if args.workload == "c3":
+        args.family = "cpydock"
+        log("generating 8,000-atom receptor / 3,000-atom ligand (radii 1.5-2.0 A)")
+        cx = synth.make_config("c3")
+    else:
+        log("generating 12,625-atom receptor / 1,500-atom ligand")
+        cx = synth.make_config("c2")
     t = time.perf_counter()
     cx.build_field()
     log("phi %s built on GPU in %.2f s" % (cx.dims, time.perf_counter() - t))
@@ -124,7 +146,7 @@ def main():
     ctx = lklight.Context(cx, desolv=desolv)
     flags = lklight.family_flags(args.family) | (lklight.F_DESOLV if desolv else 0)
 
-    K, W, N = args.steps, args.warmup, POSES_PER_STEP
+    K, W, N = args.steps, args.warmup, (400 if args.workload == "c3" else POSES_PER_STEP)
     poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
     tune = None
     if args.cell or args.block or args.chunk >= 0:
@@ -204,14 +226,19 @@ def main():
         "metric": METRIC, "value": value, "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": "c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "
-                               "(6 swarms x 200 glowworms), phi %s at 0.5 A"
-                               % (args.family, N, "x".join(map(str, cx.dims))),
+        "config": {"workload": ("c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "
+                                "(6 swarms x 200 glowworms), phi %s at 0.5 A"
+                                % (args.family, N, "x".join(map(str, cx.dims))))
+                   if args.workload == "c2" else
+                   ("c3-cpydock: 8,000-atom receptor x 3,000-atom ligand (radii 1.5-2.0 A), %d poses/step, "
+                    "contact-SASA desolvation, phi %s at 0.5 A" % (N, "x".join(map(str, cx.dims)))),
                    "family": args.family, "flags": flags, "poses_per_step": N,
                    "cell_edge_A": tiles[0], "block": tiles[1], "ligand_atoms_per_tile": tiles[2],
                    "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                     "traffic": measured_traffic(args.workload, args.family, N),
+                     "traffic_source": TRAFFIC_FILE if measured_traffic(args.workload, args.family, N) else None,
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_pose": b_pose,
                      "n_near_per_pose": n_near, "n_far_per_pose": n_far},
         "cpu_baseline": cpu,

@@ -306,3 +306,23 @@ def test_config2_full_size(gpu):
     again = run(ctx, poses[600:], lklight.FLAGS_DNA)
     assert np.array_equal(again["e"], full["e"][600:]) and np.array_equal(again["v"], full["v"][600:])
     ctx.close()
+
+
+def test_config3_cpydock(gpu):
+    """8,000-atom receptor / 3,000-atom ligand (radii 1.5-2.0 A) with contact-SASA
+    desolvation at full size (per-pose minima in LDS): oracle on a subset of a
+    400-pose step."""
+    cx = synth.make_config("c3")
+    say("c3 field build (GPU)")
+    cx.build_field()
+    poses = synth.make_poses(400, cx.meta["rec_axes"], 3)
+    ctx = lklight.Context(cx, desolv=True)
+    flags = lklight.FLAGS_PYDOCK | lklight.F_DESOLV
+    say("c3 score 400 poses")
+    full = run(ctx, poses, flags)
+    sub = poses[::25]
+    say("c3 oracle subset", sub.shape[0])
+    assert_scores("cpydock", {k: full[k][::25] for k in ("e", "v", "s")},
+                  O.Oracle(cx).batch("cpydock", sub, nthreads=8))
+    assert np.abs(full["s"]).max() > 0
+    ctx.close()
