@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--cell", type=float, default=0.0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
+    ap.add_argument("--cell-x", type=float, default=0.0, help="receptor cell edge along x (0 = cubic)")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 12.6k-atom metric workload; c3: 8k x 3k cpydock, 400 poses/step; "
@@ -149,18 +150,20 @@ def main():
     K, W, N = args.steps, args.warmup, (400 if args.workload == "c3" else POSES_PER_STEP)
     poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
     tune = None
-    if args.cell or args.block or args.chunk >= 0:
-        tiles = (args.cell or 4.0, args.block or 256, max(args.chunk, 0))
-        ctx.set_tiles(*tiles)
+    if args.cell or args.block or args.chunk >= 0 or args.cell_x:
+        tiles = (args.cell or 5.0, args.block or 256, max(args.chunk, 0), args.cell_x or (args.cell or 5.0))
+        ctx.set_tiles(tiles[0], tiles[1], tiles[2], cell_x=args.cell_x)
     elif not args.no_tune:
         t = time.perf_counter()
-        cell, block, chunk, table = ctx.autotune(poses[:N], flags)
-        tiles = (cell, block, chunk)
-        tune = {"table": [[round(a, 2), int(b), int(c), round(d, 4)] for a, b, c, d in table],
+        cell, cell_x, block, chunk, table = ctx.autotune(poses[:N], flags)
+        tiles = (cell, block, chunk, cell_x)
+        tune = {"table": [[round(a, 2), round(b, 2), int(c), int(d), round(e, 4)] for a, b, c, d, e in table],
+                "columns": ["cell_A", "cell_x_A", "block", "chunk", "ms"],
                 "seconds": round(time.perf_counter() - t, 2)}
-        log("autotuner: cell %.1f A, block %d, chunk %d (%.1f s)" % (cell, block, chunk, tune["seconds"]))
+        log("autotuner: cell %.2f A (x %.2f A), block %d, chunk %d (%.1f s)"
+            % (cell, cell_x, block, chunk, tune["seconds"]))
     else:
-        tiles = (5.0, 256, 0)
+        tiles = (5.0, 256, 0, 5.0)
     d_poses = torch.from_numpy(poses).to(dev)
     d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
     d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
@@ -233,7 +236,8 @@ def main():
                    ("c3-cpydock: 8,000-atom receptor x 3,000-atom ligand (radii 1.5-2.0 A), %d poses/step, "
                     "contact-SASA desolvation, phi %s at 0.5 A" % (N, "x".join(map(str, cx.dims)))),
                    "family": args.family, "flags": flags, "poses_per_step": N,
-                   "cell_edge_A": tiles[0], "block": tiles[1], "ligand_atoms_per_tile": tiles[2],
+                   "cell_edge_A": tiles[0], "cell_edge_x_A": tiles[3], "block": tiles[1],
+                   "ligand_atoms_per_tile": tiles[2],
                    "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,

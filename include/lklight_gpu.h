@@ -155,15 +155,20 @@ int lkl_ctx_transform(lkl_ctx* ctx, const double* poses, int N, double* out_xyz)
  * later score calls.  Scores are bitwise identical for every setting. */
 int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block, int chunk);
 
-/* Autotuner: time every (receptor cell edge, workgroup size, ligand tile)
- * candidate on N host poses with `flags`, keep the fastest in the context and
- * report it.  Scores are bitwise identical for every candidate.  table (may be
- * NULL) receives up to 128 rows of (cell A, block, chunk, median ms); *n_table
- * their count.  No reference counterpart (full_score.cu:442 hard-codes one
- * 256-thread shape). */
+/* Receptor cell edge along x (A, in (0.5, 20]; 0 = cell_size, cubic cells).
+ * Narrower x cells trim each cell row's candidate range.  Scores are bitwise
+ * identical for every setting. */
+int lkl_ctx_set_cell_x(lkl_ctx* ctx, float cell_x);
+
+/* Autotuner: time every (cell edge y/z, cell edge x, workgroup size, ligand
+ * tile) candidate on N host poses with `flags`, keep the fastest in the context
+ * and report it.  Scores are bitwise identical for every candidate.  table
+ * (may be NULL) receives up to 256 rows of (cell A, cell_x A, block, chunk,
+ * median ms); *n_table their count.  No reference counterpart
+ * (full_score.cu:442 hard-codes one 256-thread shape). */
 int lkl_ctx_autotune(lkl_ctx* ctx, const double* poses, int N, unsigned flags,
-                     float* best_cell, int* best_block, int* best_chunk, float* table,
-                     int* n_table);
+                     float* best_cell, float* best_cell_x, int* best_block, int* best_chunk,
+                     float* table, int* n_table);
 
 /* Build the context's far field on the device from its receptor coordinates
  * and charges (ReceptorField::build, grid_dna.rs:44-149, the reference's values

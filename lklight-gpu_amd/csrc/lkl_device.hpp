@@ -52,8 +52,9 @@ struct DevComplex {
     const float* rdes;
     const float* rasa;
     int nr, ncx, ncy, ncz;
-    double cox, coy, coz, inv_cs;  // cell-grid origin and 1/edge
-    float rwin, rwin2;              // window radius in cells and its square
+    double cox, coy, coz;           // cell-grid origin (A)
+    float csyz, inv_csyz, inv_csx;  // cell edge along y and z, 1/edge (y, z) and 1/edge (x)
+    float rwin, rwin2;              // window radius (A) and its square
     // ligand, spatially sorted (Morton order of the reference coordinates)
     const float4* lxyzq;
     const float4* lpar;

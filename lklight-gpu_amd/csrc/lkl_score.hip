@@ -101,23 +101,25 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
     const float xl = (float)(x64 - (double)xh);
     const float yl = (float)(y64 - (double)yh);
     const float zl = (float)(z64 - (double)zh);
-    // position in cell units (window arithmetic only; margins cover f32 error)
-    const float fx = fminf(fmaxf((float)((x64 - C.cox) * C.inv_cs), -1.0e4f), 1.0e4f);
-    const float fy = fminf(fmaxf((float)((y64 - C.coy) * C.inv_cs), -1.0e4f), 1.0e4f);
-    const float fz = fminf(fmaxf((float)((z64 - C.coz) * C.inv_cs), -1.0e4f), 1.0e4f);
-    const float rw = C.rwin, rw2 = C.rwin2;
-    const int izlo = max(0, (int)floorf(fz - rw)), izhi = min(C.ncz - 1, (int)floorf(fz + rw));
-    const int iylo = max(0, (int)floorf(fy - rw)), iyhi = min(C.ncy - 1, (int)floorf(fy + rw));
+    // position relative to the cell-grid origin, in A (window arithmetic only;
+    // the window radius's margin covers f32 error).  Cells are cs x cs in y/z
+    // and may be narrower along x, which trims each row's x-range.
+    const float px = fminf(fmaxf((float)(x64 - C.cox), -1.0e4f), 1.0e4f);
+    const float py = fminf(fmaxf((float)(y64 - C.coy), -1.0e4f), 1.0e4f);
+    const float pz = fminf(fmaxf((float)(z64 - C.coz), -1.0e4f), 1.0e4f);
+    const float rw = C.rwin, rw2 = C.rwin2, cs = C.csyz, ics = C.inv_csyz;
+    const int izlo = max(0, (int)floorf((pz - rw) * ics)), izhi = min(C.ncz - 1, (int)floorf((pz + rw) * ics));
+    const int iylo = max(0, (int)floorf((py - rw) * ics)), iyhi = min(C.ncy - 1, (int)floorf((py + rw) * ics));
     for (int iz = izlo; iz <= izhi; ++iz) {
-        const float gz = fmaxf(0.0f, fmaxf((float)iz - fz, fz - (float)(iz + 1)));
+        const float gz = fmaxf(0.0f, fmaxf((float)iz * cs - pz, pz - (float)(iz + 1) * cs));
         const float gz2 = gz * gz;
         for (int iy = iylo; iy <= iyhi; ++iy) {
-            const float gy = fmaxf(0.0f, fmaxf((float)iy - fy, fy - (float)(iy + 1)));
+            const float gy = fmaxf(0.0f, fmaxf((float)iy * cs - py, py - (float)(iy + 1) * cs));
             const float g2 = gz2 + gy * gy;
             if (g2 > rw2) continue;
             const float b = SQRT_W(rw2 - g2);
-            const int xlo = max(0, (int)floorf(fx - b));
-            const int xhi = min(C.ncx - 1, (int)floorf(fx + b));
+            const int xlo = max(0, (int)floorf((px - b) * C.inv_csx));
+            const int xhi = min(C.ncx - 1, (int)floorf((px + b) * C.inv_csx));
             if (xlo > xhi) continue;
             const int row = (iz * C.ncy + iy) * C.ncx;
             const int pb = C.cstart[row + xlo];
@@ -404,7 +406,8 @@ struct lkl_ctx {
     size_t lds_max = 65536;
     int block = 256;
     int chunk = 0;  // ligand atoms per workgroup tile (0 = the whole ligand)
-    float cell_size = 5.0f;
+    float cell_size = 5.0f;  // cell edge along y and z (A)
+    float cell_x = 0.0f;     // cell edge along x (A); 0 = cell_size
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -467,9 +470,11 @@ static int dev_reserve(T** buf, size_t* cap, size_t n) {
 
 // Receptor layout: uniform cells of edge cell_size over the receptor's box,
 // atoms stably counting-sorted by cell (x fastest), packed float4 records.
+static float cell_x_of(const lkl_ctx* c) { return c->cell_x > 0.0f ? c->cell_x : c->cell_size; }
+
 static int build_receptor(lkl_ctx* c) {
     const int nr = c->nr;
-    const double cs = c->cell_size;
+    const double edge[3] = {(double)cell_x_of(c), (double)c->cell_size, (double)c->cell_size};
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     if (nr > 0) {
         for (int a = 0; a < 3; ++a) { lo[a] = INFINITY; hi[a] = -INFINITY; }
@@ -482,7 +487,7 @@ static int build_receptor(lkl_ctx* c) {
     int n[3];
     for (int a = 0; a < 3; ++a) {
         lo[a] = std::floor(lo[a]) - 1.0;
-        n[a] = std::max(1, (int)std::floor((hi[a] - lo[a]) / cs) + 1);
+        n[a] = std::max(1, (int)std::floor((hi[a] - lo[a]) / edge[a]) + 1);
     }
     const size_t ncell = (size_t)n[0] * n[1] * n[2];
     if (ncell > (size_t)1 << 28) { set_err("receptor cell grid too large (%zu cells)", ncell); return -1; }
@@ -490,7 +495,7 @@ static int build_receptor(lkl_ctx* c) {
     for (int i = 0; i < nr; ++i) {
         int ii[3];
         for (int a = 0; a < 3; ++a) {
-            ii[a] = (int)std::floor(((double)c->r_xyz[3 * i + a] - lo[a]) / cs);
+            ii[a] = (int)std::floor(((double)c->r_xyz[3 * i + a] - lo[a]) / edge[a]);
             ii[a] = std::min(std::max(ii[a], 0), n[a] - 1);
         }
         cell[i] = (ii[2] * n[1] + ii[1]) * n[0] + ii[0];
@@ -604,8 +609,10 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.cox = c->cox;
     C.coy = c->coy;
     C.coz = c->coz;
-    C.inv_cs = 1.0 / (double)c->cell_size;
-    C.rwin = std::sqrt(WIN2) / c->cell_size;
+    C.csyz = c->cell_size;
+    C.inv_csyz = (float)(1.0 / (double)c->cell_size);
+    C.inv_csx = (float)(1.0 / (double)cell_x_of(c));
+    C.rwin = std::sqrt(WIN2);
     C.rwin2 = C.rwin * C.rwin;
     C.lxyzq = c->d_lxyzq;
     C.lpar = c->d_lpar;
@@ -892,18 +899,32 @@ LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block, int chunk)
     return 0;
 }
 
+LKL_API int lkl_ctx_set_cell_x(lkl_ctx* c, float cell_x) {
+    if (!c) { set_err("lkl_ctx_set_cell_x: null context"); return -1; }
+    if (!(cell_x == 0.0f || (cell_x > 0.5f && cell_x <= 20.0f))) { set_err("cell_x %.3f not 0 or in (0.5, 20]", cell_x); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    if (cell_x != c->cell_x) {
+        c->cell_x = cell_x;
+        if (build_receptor(c)) return -1;
+    }
+    return 0;
+}
+
 // ── autotuner ────────────────────────────────────────────────────────────────
-// Times every (receptor cell edge, workgroup size) candidate on the caller's
-// poses with HIP events and keeps the fastest.  Per-pose results are bitwise
-// identical for every candidate (fixed-point sums), so tuning never changes a
-// score.  table (optional) receives n_cand rows of (cell, block, ms).
+// Times every (cell edge y/z, cell edge x, workgroup size, ligand tile)
+// candidate on the caller's poses with HIP events and keeps the fastest.
+// Per-pose results are bitwise identical for every candidate (fixed-point
+// sums), so tuning never changes a score.  table (optional) receives rows of
+// (cell, cell_x, block, chunk, median ms).
 static const float k_cells[] = {3.4f, 4.0f, 5.0f, 6.0f, 7.5f};
+static const float k_cell_x[] = {0.0f, 2.5f, 1.25f};  // 0 = cubic cells
 static const int k_blocks[] = {64, 128, 256, 512};
 static const int k_chunk_mult[] = {1, 2, 4, 0};  // ligand atoms per lane (0 = whole ligand)
+constexpr int TUNE_ROWS = 256;
 
 LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned flags,
-                             float* best_cell, int* best_block, int* best_chunk, float* table,
-                             int* n_table) {
+                             float* best_cell, float* best_cell_x, int* best_block, int* best_chunk,
+                             float* table, int* n_table) {
     if (!c || !poses || N <= 0) { set_err("lkl_ctx_autotune: bad arguments"); return -1; }
     if (validate_flags(c, flags)) return -1;
     HIPCK("probe", hipSetDevice(c->device));
@@ -914,39 +935,45 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
     hipEvent_t e0, e1;
     HIPCK("alloc", hipEventCreate(&e0));
     HIPCK("alloc", hipEventCreate(&e1));
-    float bc = c->cell_size, bms = INFINITY;
+    float bc = c->cell_size, bcx = c->cell_x, bms = INFINITY;
     int bb = c->block, bch = c->chunk, k = 0;
     int rc = 0;
     for (float cell : k_cells) {
-        if (cell != c->cell_size) {
-            c->cell_size = cell;
-            if (build_receptor(c)) { rc = -1; break; }
-        }
-        for (int block : k_blocks) {
-            for (int mult : k_chunk_mult) {
-                const int chunk = mult ? block * mult : 0;
-                if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile
-                if ((flags & F_DESOLV) && mult) continue;
-                c->block = block;
-                c->chunk = chunk;
-                double* dS = (flags & F_DESOLV) ? c->d_outS : nullptr;
-                if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) { rc = -1; break; }
-                float ms[3];
-                for (int r = 0; r < 3 && rc == 0; ++r) {
-                    (void)hipEventRecord(e0, c->stream);
-                    if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) rc = -1;
-                    (void)hipEventRecord(e1, c->stream);
-                    (void)hipEventSynchronize(e1);
-                    (void)hipEventElapsedTime(&ms[r], e0, e1);
+        for (float cx : k_cell_x) {
+            if (cx != 0.0f && cx >= cell) continue;  // same as (or coarser than) cubic
+            if (cell != c->cell_size || cx != c->cell_x) {
+                c->cell_size = cell;
+                c->cell_x = cx;
+                if (build_receptor(c)) { rc = -1; break; }
+            }
+            for (int block : k_blocks) {
+                for (int mult : k_chunk_mult) {
+                    const int chunk = mult ? block * mult : 0;
+                    if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile
+                    if ((flags & F_DESOLV) && mult) continue;
+                    c->block = block;
+                    c->chunk = chunk;
+                    double* dS = (flags & F_DESOLV) ? c->d_outS : nullptr;
+                    if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) { rc = -1; break; }
+                    float ms[3];
+                    for (int r = 0; r < 3 && rc == 0; ++r) {
+                        (void)hipEventRecord(e0, c->stream);
+                        if (launch_score(c, c->d_poses, N, flags, c->d_out, dS, c->stream, false)) rc = -1;
+                        (void)hipEventRecord(e1, c->stream);
+                        (void)hipEventSynchronize(e1);
+                        (void)hipEventElapsedTime(&ms[r], e0, e1);
+                    }
+                    if (rc) break;
+                    std::sort(ms, ms + 3);
+                    if (table && k < TUNE_ROWS) {
+                        float* row = table + 5 * k;
+                        row[0] = cell; row[1] = cell_x_of(c); row[2] = (float)block;
+                        row[3] = (float)chunk; row[4] = ms[1];
+                    }
+                    ++k;
+                    if (ms[1] < bms) { bms = ms[1]; bc = cell; bcx = cx; bb = block; bch = chunk; }
                 }
                 if (rc) break;
-                std::sort(ms, ms + 3);
-                if (table && k < 128) {
-                    table[4 * k] = cell; table[4 * k + 1] = (float)block;
-                    table[4 * k + 2] = (float)chunk; table[4 * k + 3] = ms[1];
-                }
-                ++k;
-                if (ms[1] < bms) { bms = ms[1]; bc = cell; bb = block; bch = chunk; }
             }
             if (rc) break;
         }
@@ -955,14 +982,16 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return -1;
-    if (n_table) *n_table = std::min(k, 128);
+    if (n_table) *n_table = std::min(k, TUNE_ROWS);
     c->block = bb;
     c->chunk = bch;
-    if (bc != c->cell_size) {
+    if (bc != c->cell_size || bcx != c->cell_x) {
         c->cell_size = bc;
+        c->cell_x = bcx;
         if (build_receptor(c)) return -1;
     }
     if (best_cell) *best_cell = bc;
+    if (best_cell_x) *best_cell_x = cell_x_of(c);
     if (best_block) *best_block = bb;
     if (best_chunk) *best_chunk = bch;
     return 0;

@@ -130,17 +130,20 @@ class Context:
         return out
 
     def autotune(self, poses, flags):
-        """lkl_ctx_autotune -> (cell, block, chunk, table[(cell, block, chunk, ms)])."""
+        """lkl_ctx_autotune -> (cell, cell_x, block, chunk,
+        table[(cell, cell_x, block, chunk, ms)])."""
         p = _capi.as_f64(poses).reshape(-1, 7)
-        cell, block, chunk, n = C.c_float(0), C.c_int(0), C.c_int(0), C.c_int(0)
-        table = np.zeros((128, 4), np.float32)
+        cell, cx, block, chunk, n = C.c_float(0), C.c_float(0), C.c_int(0), C.c_int(0), C.c_int(0)
+        table = np.zeros((256, 5), np.float32)
         _capi.check(lib().lkl_ctx_autotune(self.h, _capi.ptr(p, C.c_double), p.shape[0], flags,
-                                           C.byref(cell), C.byref(block), C.byref(chunk),
+                                           C.byref(cell), C.byref(cx), C.byref(block), C.byref(chunk),
                                            _capi.ptr(table, C.c_float), C.byref(n)),
                     "lkl_ctx_autotune")
-        return cell.value, block.value, chunk.value, table[:n.value].tolist()
+        return cell.value, cx.value, block.value, chunk.value, table[:n.value].tolist()
 
-    def set_tiles(self, cell_size, block, chunk=0):
+    def set_tiles(self, cell_size, block, chunk=0, cell_x=0.0):
+        """Cell edge (y, z; and x unless cell_x > 0), workgroup size, ligand atoms per tile."""
+        _capi.check(lib().lkl_ctx_set_cell_x(self.h, float(cell_x)), "lkl_ctx_set_cell_x")
         _capi.check(lib().lkl_ctx_set_tiles(self.h, float(cell_size), int(block), int(chunk)),
                     "lkl_ctx_set_tiles")
 

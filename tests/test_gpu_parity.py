@@ -115,23 +115,24 @@ def test_cpydock_parity(small):
 
 
 def test_tile_and_order_invariance(small):
-    """Fixed-point per-pose sums: bitwise identical for every cell edge, block
-    size, and input atom order."""
+    """Fixed-point per-pose sums: bitwise identical for every cell edge (cubic
+    and narrower along x), block size, ligand tile, and input atom order."""
     cx, orc, poses = small
     say("tile invariance")
     ctx = lklight.Context(cx, desolv=True)
     flags = lklight.FLAGS_DNA
     base = run(ctx, poses, flags)
     base_c = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
-    for cell, block, chunk in [(10.0, 256, 0), (3.4, 64, 64), (6.0, 512, 0), (4.0, 128, 256),
-                               (5.0, 64, 128)]:
-        ctx.set_tiles(cell, block, chunk)
+    for cell, block, chunk, cell_x in [(10.0, 256, 0, 0.0), (3.4, 64, 64, 0.0), (6.0, 512, 0, 0.0),
+                                       (4.0, 128, 256, 0.0), (5.0, 64, 128, 0.0), (5.0, 256, 0, 1.25),
+                                       (7.5, 128, 64, 2.5), (3.4, 512, 0, 0.75)]:
+        ctx.set_tiles(cell, block, chunk, cell_x=cell_x)
         r = run(ctx, poses, flags)
         rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         for k in ("e", "v"):
-            assert np.array_equal(r[k], base[k]), (cell, block, chunk, k)
+            assert np.array_equal(r[k], base[k]), (cell, block, chunk, cell_x, k)
         for k in ("e", "v", "s"):
-            assert np.array_equal(rc[k], base_c[k]), (cell, block, chunk, k)
+            assert np.array_equal(rc[k], base_c[k]), (cell, block, chunk, cell_x, k)
     ctx.close()
     say("order invariance")
     rng = np.random.default_rng(0)
