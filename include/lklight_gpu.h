@@ -160,6 +160,13 @@ int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block, int chunk);
  * identical for every setting. */
 int lkl_ctx_set_cell_x(lkl_ctx* ctx, float cell_x);
 
+/* Work ordering of the near field (default on): each tile's ligand atoms are
+ * bucketed by the receptor-atom count near them (a 2 A receptor grid built at
+ * creation) so a wavefront's lanes walk windows of similar size, and atoms with
+ * no receptor atom in reach skip the near-field walk.  Scores are bitwise
+ * identical either way. */
+int lkl_ctx_set_work_sort(lkl_ctx* ctx, int on);
+
 /* Autotuner: time every (cell edge y/z, cell edge x, workgroup size, ligand
  * tile) candidate on N host poses with `flags`, keep the fastest in the context
  * and report it.  Scores are bitwise identical for every candidate.  table

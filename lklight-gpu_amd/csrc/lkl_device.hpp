@@ -55,6 +55,11 @@ struct DevComplex {
     double cox, coy, coz;           // cell-grid origin (A)
     float csyz, inv_csyz, inv_csx;  // cell edge along y and z, 1/edge (y, z) and 1/edge (x)
     float rwin, rwin2;              // window radius (A) and its square
+    // work grid: receptor atoms within rwin + (node half-diagonal) of each
+    // node (NULL = unsorted near field); nodes at wo + wh * i
+    const unsigned short* wgrid;
+    int wnx, wny, wnz;
+    float wox, woy, woz, inv_wh;
     // ligand, spatially sorted (Morton order of the reference coordinates)
     const float4* lxyzq;
     const float4* lpar;

@@ -191,6 +191,103 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
 }
 
 
+// ── work grid ────────────────────────────────────────────────────────────────
+// Per node (spacing WORK_H over the receptor box grown by WORK_PAD): the number
+// of receptor atoms within the window radius plus the node's half-diagonal
+// (+ margin), so a ligand atom whose nearest node counts 0 has no receptor atom
+// in its window.  Used to order each tile's ligand atoms by expected work and
+// to skip those with none; it never changes a sum.
+constexpr float WORK_H = 2.0f, WORK_PAD = 12.0f;
+constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (48 KB of LDS lists)
+
+__global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz,
+                                int nx, int ny, long long nodes, float r2, unsigned short* __restrict__ out) {
+    const long long node = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (node >= nodes) return;
+    const int ix = (int)(node % nx);
+    const long long t = node / nx;
+    const int iy = (int)(t % ny), iz = (int)(t / ny);
+    const float px = ox + (float)ix * WORK_H, py = oy + (float)iy * WORK_H, pz = oz + (float)iz * WORK_H;
+    unsigned cnt = 0;
+    for (int i = 0; i < nr; ++i) {
+        const float4 r = rec[i];
+        const float dx = px - r.x, dy = py - r.y, dz = pz - r.z;
+        cnt += (dx * dx + dy * dy + dz * dz <= r2) ? 1u : 0u;
+    }
+    out[node] = (unsigned short)min(cnt, 65535u);
+}
+
+// Bucket of a work count: NBKT (last) for no work, else descending
+// quarter-octaves of the count (larger counts first).
+#ifndef LKL_NBKT
+#define LKL_NBKT 16
+#endif
+constexpr int NBKT = LKL_NBKT;
+static_assert(NBKT >= 1 && NBKT <= 254, "bucket ids are stored in one byte");
+__device__ __forceinline__ int work_bucket(unsigned w) {
+    if (w == 0u) return NBKT;
+    const int q = (int)(__log2f((float)w) * (float)(NBKT / 16));
+    return NBKT - 1 - min(NBKT - 1, max(0, q));
+}
+
+__device__ __forceinline__ unsigned work_at(const DevComplex& C, double x, double y, double z) {
+    const int ix = (int)floorf(((float)x - C.wox) * C.inv_wh + 0.5f);
+    const int iy = (int)floorf(((float)y - C.woy) * C.inv_wh + 0.5f);
+    const int iz = (int)floorf(((float)z - C.woz) * C.inv_wh + 0.5f);
+    if (ix < 0 || iy < 0 || iz < 0 || ix >= C.wnx || iy >= C.wny || iz >= C.wnz) return 0u;
+    return C.wgrid[((size_t)iz * C.wny + iy) * C.wnx + ix];
+}
+
+// Near field (+ far field) of ligand atoms [j0, j1) of one pose, work-ordered:
+// pass 1 transforms every atom, adds its far field and files it in a bucket
+// (descending work-grid count; NBKT = none); pass 2 walks the windows of the
+// atoms with work in bucket order, so the lanes of a wavefront walk windows of
+// similar size.  All threads of the block must call it (barriers); lds holds
+// (j1 - j0) * 3 bytes.
+template <bool FAR, bool ELEC, bool CLASH>
+__device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
+                                            long long& e, long long& v) {
+    __shared__ unsigned bcnt[NBKT + 1], boff[NBKT + 1];
+    const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+    const int nT = j1 - j0;
+    unsigned short* list = reinterpret_cast<unsigned short*>(lds);
+    unsigned char* bkt = reinterpret_cast<unsigned char*>(list + nT);
+    for (int k = tid; k <= NBKT; k += bs) bcnt[k] = 0u;
+    __syncthreads();
+    for (int t = tid; t < nT; t += bs) {
+        const float4 lb = C.lxyzq[j0 + t];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
+        const int bb = work_bucket(work_at(C, x, y, z));
+        bkt[t] = (unsigned char)bb;
+        atomicAdd(&bcnt[bb], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned acc0 = 0u;
+        for (int k = 0; k <= NBKT; ++k) {
+            boff[k] = acc0;
+            acc0 += bcnt[k];
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < nT; t += bs) list[atomicAdd(&boff[bkt[t]], 1u)] = (unsigned short)t;
+    __syncthreads();
+    const int nact = nT - (int)bcnt[NBKT];
+    for (int k = tid; k < nact; k += bs) {
+        const int j = j0 + list[k];
+        const float4 lb = C.lxyzq[j];
+        const float4 lp = C.lpar[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        double minL = MIN_INIT;
+        near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v, minL,
+                                         nullptr);
+    }
+    __syncthreads();  // the LDS lists may be reused by the caller
+}
+
 // One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
 // out[2k+1] = V, outS[k] = S.  With several chunks per pose the tiles add their
 // fixed-point partials into acc[3k..3k+2] (integer atomics: exact, order-free);
@@ -228,6 +325,22 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     }
 
     long long e = 0, v = 0, s = 0;
+    if constexpr (DES == DES_NONE && !PERATOM) {
+        if (C.wgrid) {
+            near_sorted<FAR, ELEC, CLASH>(C, M, j0, j1, lds_min, e, v);
+        } else {
+            for (int j = j0 + tid; j < j1; j += bs) {
+                const float4 lb = C.lxyzq[j];
+                const float4 lp = C.lpar[j];
+                double x, y, z;
+                transform(M, lb.x, lb.y, lb.z, x, y, z);
+                if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
+                double minL = MIN_INIT;
+                near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
+                                                 minL, nullptr);
+            }
+        }
+    } else
     for (int j = j0 + tid; j < j1; j += bs) {
         const float4 lb = C.lxyzq[j];
         const float4 lp = C.lpar[j];
@@ -296,8 +409,12 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
 // Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
 // thread's share (no desolvation, no per-atom output).
 template <bool FAR, bool ELEC, bool CLASH>
-__device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1,
+__device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
+    if (C.wgrid) {
+        near_sorted<FAR, ELEC, CLASH>(C, M, j0, j1, lds, e, v);
+        return;
+    }
     for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
         const float4 lb = C.lxyzq[j];
         const float4 lp = C.lpar[j];
@@ -332,6 +449,7 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
              int total_tiles, const double* __restrict__ poses, double* __restrict__ out,
              long long* __restrict__ acc, unsigned* __restrict__ cnt) {
     __shared__ long long red[2][16];
+    extern __shared__ __attribute__((aligned(16))) unsigned char multi_lds[];  // work-ordered lists
     const int t = (int)blockIdx.x;
     if (t >= total_tiles) return;
     const int tid = threadIdx.x, bs = blockDim.x;
@@ -350,14 +468,14 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
     long long e = 0, v = 0;
     switch (P.flags & 7u) {
-        case 0: tile_sums<false, false, false>(C, M, j0, j1, e, v); break;
-        case 1: tile_sums<true, false, false>(C, M, j0, j1, e, v); break;
-        case 2: tile_sums<false, true, false>(C, M, j0, j1, e, v); break;
-        case 3: tile_sums<true, true, false>(C, M, j0, j1, e, v); break;
-        case 4: tile_sums<false, false, true>(C, M, j0, j1, e, v); break;
-        case 5: tile_sums<true, false, true>(C, M, j0, j1, e, v); break;
-        case 6: tile_sums<false, true, true>(C, M, j0, j1, e, v); break;
-        default: tile_sums<true, true, true>(C, M, j0, j1, e, v); break;
+        case 0: tile_sums<false, false, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 1: tile_sums<true, false, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 2: tile_sums<false, true, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 3: tile_sums<true, true, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 4: tile_sums<false, false, true>(C, M, j0, j1, multi_lds, e, v); break;
+        case 5: tile_sums<true, false, true>(C, M, j0, j1, multi_lds, e, v); break;
+        case 6: tile_sums<false, true, true>(C, M, j0, j1, multi_lds, e, v); break;
+        default: tile_sums<true, true, true>(C, M, j0, j1, multi_lds, e, v); break;
     }
     e = wave_sum(e);
     v = wave_sum(v);
@@ -408,6 +526,10 @@ struct lkl_ctx {
     int chunk = 0;  // ligand atoms per workgroup tile (0 = the whole ligand)
     float cell_size = 5.0f;  // cell edge along y and z (A)
     float cell_x = 0.0f;     // cell edge along x (A); 0 = cell_size
+    bool work_sort = true;   // order each tile's ligand atoms by near-field work
+    unsigned short* d_wgrid = nullptr;
+    int wn[3] = {0, 0, 0};
+    double wo[3] = {0, 0, 0};
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -528,6 +650,31 @@ static int build_receptor(lkl_ctx* c) {
         if (dev_upload(&c->d_rdes, des.data(), (size_t)nr)) return -1;
         if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr)) return -1;
     }
+    if (!c->d_wgrid && nr > 0) {  // receptor-only: built once
+        double wlo[3], whi[3];
+        long long nodes = 1;
+        for (int a = 0; a < 3; ++a) {
+            wlo[a] = INFINITY;
+            whi[a] = -INFINITY;
+        }
+        for (int i = 0; i < nr; ++i)
+            for (int a = 0; a < 3; ++a) {
+                wlo[a] = std::min(wlo[a], (double)c->r_xyz[3 * i + a]);
+                whi[a] = std::max(whi[a], (double)c->r_xyz[3 * i + a]);
+            }
+        for (int a = 0; a < 3; ++a) {
+            c->wo[a] = std::floor(wlo[a] - WORK_PAD);
+            c->wn[a] = (int)std::ceil((whi[a] + WORK_PAD - c->wo[a]) / WORK_H) + 1;
+            nodes *= c->wn[a];
+        }
+        const float rw = std::sqrt(WIN2) + 0.5f * std::sqrt(3.0f) * WORK_H + 0.01f;
+        HIPCK("alloc", hipMalloc((void**)&c->d_wgrid, sizeof(unsigned short) * (size_t)nodes));
+        hipLaunchKernelGGL(workgrid_kernel, dim3((unsigned)((nodes + 255) / 256)), dim3(256), 0, c->stream,
+                           c->d_rxyzq, nr, (float)c->wo[0], (float)c->wo[1], (float)c->wo[2], c->wn[0],
+                           c->wn[1], nodes, rw * rw, c->d_wgrid);
+        HIPCK("launch", hipGetLastError());
+        HIPCK("sync", hipStreamSynchronize(c->stream));
+    }
     c->version++;
     c->ncx = n[0];
     c->ncy = n[1];
@@ -614,6 +761,14 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.inv_csx = (float)(1.0 / (double)cell_x_of(c));
     C.rwin = std::sqrt(WIN2);
     C.rwin2 = C.rwin * C.rwin;
+    C.wgrid = c->work_sort ? c->d_wgrid : nullptr;
+    C.wnx = c->wn[0];
+    C.wny = c->wn[1];
+    C.wnz = c->wn[2];
+    C.wox = (float)c->wo[0];
+    C.woy = (float)c->wo[1];
+    C.woz = (float)c->wo[2];
+    C.inv_wh = 1.0f / WORK_H;
     C.lxyzq = c->d_lxyzq;
     C.lpar = c->d_lpar;
     C.lflag = c->d_lflag;
@@ -637,7 +792,7 @@ static void ctx_free(lkl_ctx* c) {
     void* bufs[] = {c->d_phi, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
-                    c->d_acc, c->d_cnt};
+                    c->d_acc, c->d_cnt, c->d_wgrid};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -766,6 +921,10 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     else if (des == DES_GLOBAL) fn = pick3<DES_GLOBAL, false>(flags);
     else fn = pick3<DES_NONE, false>(flags);
     int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
+    if (des == DES_NONE && !peratom && c->work_sort && c->d_wgrid) {
+        if (chunk > WORK_TILE_MAX) chunk = WORK_TILE_MAX;  // LDS list bound
+        shm = (size_t)chunk * 3 + 16;                      // index list + bucket ids
+    }
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
@@ -899,6 +1058,15 @@ LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block, int chunk)
     return 0;
 }
 
+LKL_API int lkl_ctx_set_work_sort(lkl_ctx* c, int on) {
+    if (!c) { set_err("lkl_ctx_set_work_sort: null context"); return -1; }
+    if (c->work_sort != (on != 0)) {
+        c->work_sort = on != 0;
+        c->version++;  // batches built on this context refresh their views
+    }
+    return 0;
+}
+
 LKL_API int lkl_ctx_set_cell_x(lkl_ctx* c, float cell_x) {
     if (!c) { set_err("lkl_ctx_set_cell_x: null context"); return -1; }
     if (!(cell_x == 0.0f || (cell_x > 0.5f && cell_x <= 20.0f))) { set_err("cell_x %.3f not 0 or in (0.5, 20]", cell_x); return -1; }
@@ -1005,6 +1173,7 @@ struct lkl_multi {
     std::vector<unsigned> versions;
     int K = 0, P = 0, total_tiles = 0, device = 0, cus = 256;
     int block = 256, chunk_mult = 2;
+    size_t lds = 0;  // dynamic LDS of the work-ordered tiles (largest chunk x 3 bytes)
     hipStream_t stream = nullptr;
     DevComplex* d_cx = nullptr;
     ProblemDesc* d_prob = nullptr;
@@ -1043,11 +1212,16 @@ static int batch_refresh(lkl_multi* b) {
     std::vector<DevComplex> cx(b->K);
     std::vector<ProblemDesc> pd(b->K);
     int tile = 0;
+    b->lds = 0;
     for (int r = 0; r < b->K; ++r) {
         const int k = b->order[r];
         lkl_ctx* c = b->ctxs[k];
         cx[r] = dev_view(c);
-        const int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
+        int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
+        if (cx[r].wgrid) {
+            chunk = std::min(chunk, WORK_TILE_MAX);  // LDS list bound
+            b->lds = std::max(b->lds, (size_t)chunk * 3 + 16);
+        }
         pd[r].tile_begin = tile;
         pd[r].nchunks = (c->nl + chunk - 1) / chunk;
         pd[r].chunk = chunk;
@@ -1071,7 +1245,7 @@ static int batch_check(lkl_multi* b) {
 
 static int batch_launch(lkl_multi* b, const double* d_poses, double* d_out, hipStream_t st) {
     if (b->total_tiles <= 0) return 0;
-    hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), 0, st,
+    hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, st,
                        b->d_cx, b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt);
     HIPCK("launch", hipGetLastError());
     return 0;
@@ -1171,7 +1345,7 @@ LKL_API int lkl_multi_score(lkl_multi* b, const double* poses, double* out) {
         hipError_t e1 = hipMemcpyAsync(b->d_poses, b->h_poses, sizeof(double) * 7 * P,
                                        hipMemcpyHostToDevice, b->stream);
         if (b->total_tiles > 0)
-            hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), 0,
+            hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds,
                                b->stream, b->d_cx, b->d_prob, b->K, b->total_tiles, b->d_poses,
                                b->d_out, b->d_acc, b->d_cnt);
         hipError_t e2 = hipGetLastError();

@@ -859,13 +859,17 @@ LKL_API lkl_mlp* lkl_mlp_create(unsigned long long seed) {
     return n;
 }
 
+// The object's stream is non-blocking: copies of its buffers first wait for
+// the work queued on it (an Adam step may still be running).
 LKL_API int lkl_mlp_get_params(lkl_mlp* n, float* out) {
     if (!n || !out) return -1;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return -1;
     return hipMemcpy(out, n->p, NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
 
 LKL_API int lkl_mlp_set_params(lkl_mlp* n, const float* in) {
     if (!n || !in) return -1;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return -1;
     if (hipMemcpy(n->p, in, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
     hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, n->stream, n->p, n->p16, NPARAM);
     return hipStreamSynchronize(n->stream) == hipSuccess ? 0 : -1;
@@ -873,6 +877,7 @@ LKL_API int lkl_mlp_set_params(lkl_mlp* n, const float* in) {
 
 LKL_API int lkl_mlp_get_grads(lkl_mlp* n, float* out) {
     if (!n || !out) return -1;
+    if (hipStreamSynchronize(n->stream) != hipSuccess) return -1;
     return hipMemcpy(out, n->g, NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
 
@@ -988,6 +993,7 @@ LKL_API int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* 
     if (!rc && (hipMemcpy(dx, x, (size_t)B * NF * 2, hipMemcpyHostToDevice) ||
                 hipMemcpy(dt, t, (size_t)B * 4, hipMemcpyHostToDevice))) rc = -1;
     if (!rc) rc = lkl_mlp_grad(n, dx, dt, B, loss);
+    if (hipStreamSynchronize(n->stream) != hipSuccess) rc = -1;  // before the inputs are freed
     if (dx) (void)hipFree(dx);
     if (dt) (void)hipFree(dt);
     return rc;

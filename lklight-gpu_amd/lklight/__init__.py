@@ -141,6 +141,10 @@ class Context:
                     "lkl_ctx_autotune")
         return cell.value, cx.value, block.value, chunk.value, table[:n.value].tolist()
 
+    def set_work_sort(self, on):
+        """lkl_ctx_set_work_sort: order each tile's near field by expected work."""
+        _capi.check(lib().lkl_ctx_set_work_sort(self.h, int(bool(on))), "lkl_ctx_set_work_sort")
+
     def set_tiles(self, cell_size, block, chunk=0, cell_x=0.0):
         """Cell edge (y, z; and x unless cell_x > 0), workgroup size, ligand atoms per tile."""
         _capi.check(lib().lkl_ctx_set_cell_x(self.h, float(cell_x)), "lkl_ctx_set_cell_x")
