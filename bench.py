@@ -226,7 +226,9 @@ def main():
     b_pose = pose_bytes(cx.lig.n, n_far or 0.0, n_near or 0.0, desolv, cx.rec.n)
     achieved = b_pose * N / avg_kernel_s / 1e9 if n_near is not None else None
     out = {
-        "metric": METRIC, "value": value, "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
+        "metric": METRIC if (args.workload, args.family) == ("c2", "dna")
+        else "pose-evals/s (%s, %s)" % (args.family, "8k-atom rec" if args.workload == "c3" else "12.6k-atom rec"),
+        "value": value, "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": ("c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "

@@ -242,11 +242,13 @@ __device__ __forceinline__ unsigned work_at(const DevComplex& C, double x, doubl
 // pass 1 transforms every atom, adds its far field and files it in a bucket
 // (descending work-grid count; NBKT = none); pass 2 walks the windows of the
 // atoms with work in bucket order, so the lanes of a wavefront walk windows of
-// similar size.  All threads of the block must call it (barriers); lds holds
-// (j1 - j0) * 3 bytes.
-template <bool FAR, bool ELEC, bool CLASH>
+// similar size.  With desolvation (whole-ligand tiles) the per-atom ligand term
+// is added in whichever pass finishes the atom.  All threads of the block must
+// call it (barriers); lds holds (j1 - j0) * 3 bytes.
+template <bool FAR, bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
-                                            long long& e, long long& v) {
+                                            unsigned* __restrict__ minR, long long& e, long long& v,
+                                            long long& s) {
     __shared__ unsigned bcnt[NBKT + 1], boff[NBKT + 1];
     const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
     const int nT = j1 - j0;
@@ -262,6 +264,9 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
         const int bb = work_bucket(work_at(C, x, y, z));
         bkt[t] = (unsigned char)bb;
         atomicAdd(&bcnt[bb], 1u);
+        // no receptor atom in reach: the desolvation minimum stays at its initial value
+        if (DES != DES_NONE && bb == NBKT)
+            s += to_fix(solv_term(MIN_INIT, C.lasa[j0 + t], C.ldes[j0 + t]), E_SCALE);
     }
     __syncthreads();
     if (tid == 0) {
@@ -282,8 +287,9 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
         double x, y, z;
         transform(M, lb.x, lb.y, lb.z, x, y, z);
         double minL = MIN_INIT;
-        near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v, minL,
-                                         nullptr);
+        const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
+        near_atom<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, e, v, minL, minR);
+        if (DES != DES_NONE) s += to_fix(solv_term(minL, C.lasa[j], C.ldes[j]), E_SCALE);
     }
     __syncthreads();  // the LDS lists may be reused by the caller
 }
@@ -325,10 +331,12 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     }
 
     long long e = 0, v = 0, s = 0;
-    if constexpr (DES == DES_NONE && !PERATOM) {
-        if (C.wgrid) {
-            near_sorted<FAR, ELEC, CLASH>(C, M, j0, j1, lds_min, e, v);
-        } else {
+    if (!PERATOM && C.wgrid) {
+        // lists after the receptor minima when those live in LDS
+        void* lists = DES == DES_LDS ? (void*)(lds_min + C.nr) : (void*)lds_min;
+        near_sorted<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+    } else if constexpr (DES == DES_NONE && !PERATOM) {
+        {
             for (int j = j0 + tid; j < j1; j += bs) {
                 const float4 lb = C.lxyzq[j];
                 const float4 lp = C.lpar[j];
@@ -412,7 +420,8 @@ template <bool FAR, bool ELEC, bool CLASH>
 __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
     if (C.wgrid) {
-        near_sorted<FAR, ELEC, CLASH>(C, M, j0, j1, lds, e, v);
+        long long s_unused = 0;
+        near_sorted<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
     }
     for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
@@ -898,9 +907,12 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
                         double* d_outS, hipStream_t st, bool peratom) {
     if (N <= 0) return 0;
     if (validate_flags(c, flags)) return -1;
-    const DevComplex C = dev_view(c);
+    DevComplex C = dev_view(c);
     int des = DES_NONE;
     size_t shm = 0;
+    // work-ordered near field: tile-local lists of (chunk x 3) bytes in LDS
+    const bool want_sort = C.wgrid != nullptr && !peratom;
+    const size_t list_nl = (size_t)c->nl * 3 + 16;
     if (flags & F_DESOLV) {
         if (!d_outS) { set_err("F_DESOLV requires an outS buffer"); return -1; }
         const size_t need = (size_t)c->nr * sizeof(unsigned);
@@ -912,6 +924,9 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
             des = DES_GLOBAL;
             if (dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
         }
+        // desolvation tiles hold the whole ligand: sort only if its list fits
+        if (want_sort && c->nl <= WORK_TILE_MAX && shm + list_nl + 1024 <= c->lds_max) shm += list_nl;
+        else C.wgrid = nullptr;
     }
     if (peratom && dev_reserve(&c->d_pa_e, &c->cap_pa, (size_t)N * c->nl)) return -1;
     if (peratom && dev_reserve(&c->d_pa_v, &c->cap_pa_v, (size_t)N * c->nl)) return -1;
@@ -921,10 +936,11 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     else if (des == DES_GLOBAL) fn = pick3<DES_GLOBAL, false>(flags);
     else fn = pick3<DES_NONE, false>(flags);
     int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
-    if (des == DES_NONE && !peratom && c->work_sort && c->d_wgrid) {
+    if (des == DES_NONE && want_sort) {
         if (chunk > WORK_TILE_MAX) chunk = WORK_TILE_MAX;  // LDS list bound
         shm = (size_t)chunk * 3 + 16;                      // index list + bucket ids
     }
+    if (peratom) C.wgrid = nullptr;
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;

@@ -135,9 +135,12 @@ def test_tile_and_order_invariance(small):
             assert np.array_equal(rc[k], base_c[k]), (cell, block, chunk, cell_x, k)
         ctx.set_work_sort(False)  # the near field in plain atom order
         r = run(ctx, poses, flags)
+        rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         ctx.set_work_sort(True)
         for k in ("e", "v"):
             assert np.array_equal(r[k], base[k]), ("unsorted", cell, block, chunk, cell_x, k)
+        for k in ("e", "v", "s"):
+            assert np.array_equal(rc[k], base_c[k]), ("unsorted", cell, block, chunk, cell_x, k)
     ctx.close()
     say("order invariance")
     rng = np.random.default_rng(0)
