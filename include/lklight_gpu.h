@@ -352,7 +352,7 @@ int lkl_mlp_forward_host(lkl_mlp* n, const unsigned short* x, int B, float* y);
 int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* t, int B, float* loss);
 /* C (M x N fp32) = sum_k A(m, k) B(k, n) on the matrix cores, bf16 operands
  * given as bits with element strides A(m, k) = A[m*sam + k*sak],
- * B(k, n) = B[k*sbk + n*sbn]; M, N multiples of 128, K of 32.  0 / -1. */
+ * B(k, n) = B[k*sbk + n*sbn]; M, N multiples of 128, K of 64.  0 / -1. */
 int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, long sak,
                        const unsigned short* B, long b_len, long sbk, long sbn, int M, int N,
                        int K, float* C);

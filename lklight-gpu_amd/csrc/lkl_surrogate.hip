@@ -335,12 +335,16 @@ __global__ void select_kernel(const float* __restrict__ pred, const double* __re
 
 // ── GEMM: C[M x N] = sum_k A(m, k) * B(k, n), bf16 in, fp32 accumulate ────────
 // A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn] (one of each
-// pair of strides is 1).  M, N multiples of 128, K of 32.  Epilogues:
+// pair of strides is 1).  M, N multiples of 128, K of TK (64).  Epilogues:
 //   EPI_F32:       C32[m, n] = acc
 //   EPI_BIAS_RELU: C16[m, n] = bf16(relu(acc + bias[n]))
 //   EPI_MASK:      C32[m, n] = acc * (mask16[m, n] > 0), C16 = bf16 of the same
 enum { EPI_F32 = 0, EPI_BIAS_RELU = 1, EPI_MASK = 2 };
-constexpr int TM = 128, TN = 128, TK = 32, LDK = TK + 8;
+#ifndef LKL_GEMM_TK
+#define LKL_GEMM_TK 64
+#endif
+constexpr int TM = 128, TN = 128, TK = LKL_GEMM_TK, LDK = TK + 8;
+constexpr int NCH = TM * TK / 8 / 256;  // 16-byte chunks per thread per operand tile
 
 struct GemmArgs {
     const bf16* A;
@@ -354,32 +358,58 @@ struct GemmArgs {
     const bf16* mask;
 };
 
-// 128 x 32 tile of X(r, k) = X[r * sr + k * sk] into lds[r][k] (k contiguous).
-__device__ __forceinline__ void load_tile(const bf16* __restrict__ X, long sr, long sk, int r0,
-                                          int k0, bf16 (*lds)[LDK]) {
+// Global -> register staging of a 128 x TK tile of X(r, k) = X[r * sr + k * sk]
+// (NCH 16-byte loads per thread), then register -> LDS lds[r][k] (k
+// contiguous; a k-major source is transposed on the way in).
+struct Stage {
+    uint4 v[NCH];
+};
+
+__device__ __forceinline__ void stage_load(const bf16* __restrict__ X, long sr, long sk, int r0, int k0,
+                                           Stage& st) {
     const int tid = threadIdx.x;
-    if (sk == 1) {
-        for (int c = tid; c < TM * TK / 8; c += 256) {
-            const int r = c >> 2, kc = (c & 3) * 8;
-            const uint4 v = *(const uint4*)(X + (long)(r0 + r) * sr + k0 + kc);
-            *(uint4*)&lds[r][kc] = v;
-        }
-    } else {
-        for (int c = tid; c < TM * TK / 8; c += 256) {
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+        const int c = tid + 256 * i;
+        if (sk == 1) {
+            const int r = c / (TK / 8), kc = (c % (TK / 8)) * 8;
+            st.v[i] = *(const uint4*)(X + (long)(r0 + r) * sr + k0 + kc);
+        } else {
             const int k = c >> 4, rc = (c & 15) * 8;
-            const uint4 v = *(const uint4*)(X + (r0 + rc) + (long)(k0 + k) * sk);
-            const bf16* e = (const bf16*)&v;
+            st.v[i] = *(const uint4*)(X + (r0 + rc) + (long)(k0 + k) * sk);
+        }
+    }
+}
+
+__device__ __forceinline__ void stage_store(long sk, const Stage& st, bf16 (*lds)[LDK]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+        const int c = tid + 256 * i;
+        if (sk == 1) {
+            const int r = c / (TK / 8), kc = (c % (TK / 8)) * 8;
+            *(uint4*)&lds[r][kc] = st.v[i];
+        } else {
+            const int k = c >> 4, rc = (c & 15) * 8;
+            const bf16* e = (const bf16*)&st.v[i];
 #pragma unroll
             for (int q = 0; q < 8; ++q) lds[rc + q][k] = e[q];
         }
     }
 }
 
+// 128 x 128 output tile per workgroup, 4 waves of 64 x 64 (2 x 2 MFMA tiles of
+// 32 x 32), K in steps of TK (64).  Double-buffered: the next step's tiles are read
+// from global memory into registers while the MFMAs consume the current LDS
+// buffer, then stored into the other buffer; one barrier per step.  The 1-D
+// grid is mapped so the N-tiles of one M-tile share an XCD (and its L2).
 template <int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) bf16 As[TM][LDK];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[TN][LDK];
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    __shared__ __attribute__((aligned(16))) bf16 As[2][TM][LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][TN][LDK];
+    const unsigned ntn = (unsigned)(g.N / TN), total = ntn * (unsigned)(g.M / TM);
+    const unsigned L = xcd_tile(blockIdx.x, total);
+    const int m0 = (int)(L / ntn) * TM, n0 = (int)(L % ntn) * TN;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = (w >> 1) * 64, wn = (w & 1) * 64;
     f32x16 acc[2][2];
@@ -389,23 +419,36 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-    for (int k0 = 0; k0 < g.K; k0 += TK) {
-        load_tile(g.A, g.sam, g.sak, m0, k0, As);
-        load_tile(g.B, g.sbn, g.sbk, n0, k0, Bs);  // B^T tile: rows n, k contiguous
-        __syncthreads();
+    const int nk = g.K / TK;
+    Stage sa, sb;
+    stage_load(g.A, g.sam, g.sak, m0, 0, sa);
+    stage_load(g.B, g.sbn, g.sbk, n0, 0, sb);  // B^T tile: rows n, k contiguous
+    stage_store(g.sak, sa, As[0]);
+    stage_store(g.sbk, sb, Bs[0]);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) {
+            stage_load(g.A, g.sam, g.sak, m0, (kt + 1) * TK, sa);
+            stage_load(g.B, g.sbn, g.sbk, n0, (kt + 1) * TK, sb);
+        }
 #pragma unroll
         for (int kk = 0; kk < TK; kk += 16) {
             bf16x8 a[2], b[2];
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                a[t] = *(const bf16x8*)&As[wm + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
-                b[t] = *(const bf16x8*)&Bs[wn + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
+                a[t] = *(const bf16x8*)&As[cur][wm + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
+                b[t] = *(const bf16x8*)&Bs[cur][wn + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
             }
 #pragma unroll
             for (int ti = 0; ti < 2; ++ti)
 #pragma unroll
                 for (int tj = 0; tj < 2; ++tj)
                     acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+        if (kt + 1 < nk) {  // the other buffer was last read before the previous barrier
+            stage_store(g.sak, sa, As[cur ^ 1]);
+            stage_store(g.sbk, sb, Bs[cur ^ 1]);
         }
         __syncthreads();
     }
@@ -516,6 +559,23 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g, 
         const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
         p[i] = pi;
         p16[i] = to_bf16(pi);
+    }
+}
+
+// dst[c][r] = src[r][c] (bf16, rows x cols, both multiples of 64), through a
+// 64 x 64 LDS tile so reads and writes are coalesced.
+__global__ void __launch_bounds__(256) transpose_kernel(const bf16* __restrict__ src, int rows, int cols,
+                                                        bf16* __restrict__ dst) {
+    __shared__ bf16 t[64][66];
+    const int r0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+        const int r = i >> 6, c = i & 63;
+        t[r][c] = src[(size_t)(r0 + r) * cols + c0 + c];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
+        const int c = i >> 6, r = i & 63;
+        dst[(size_t)(c0 + c) * rows + r0 + r] = t[r][c];
     }
 }
 
@@ -770,18 +830,30 @@ struct lkl_mlp {
     int device = 0;
     hipStream_t stream = nullptr;
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
-    bf16* p16 = nullptr;
+    bf16* p16 = nullptr;   // bf16 working copy, parameter layout
+    bf16* p16t = nullptr;  // W1, W2, W3 transposed ([out][in]): the forward GEMMs' K-contiguous operand
     long step = 0;
     size_t cap = 0;  // batch capacity of the activation buffers
     bf16 *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *dza = nullptr, *dzb = nullptr;  // dz ping-pong
     float *y = nullptr, *dy = nullptr, *dz32 = nullptr, *part = nullptr, *loss = nullptr;
+    bf16 *xt = nullptr, *ht = nullptr, *dzt = nullptr;  // transposed operands of the weight-gradient GEMMs
 };
 
+// Offsets of the transposed W1, W2, W3 in p16t.
+constexpr long T1 = 0, T2 = T1 + (long)NF * D1, T3 = T2 + (long)D1 * D1, NT_W = T3 + (long)D1 * D1;
+
+static void refresh_transposed(lkl_mlp* n, hipStream_t st) {
+    hipLaunchKernelGGL(transpose_kernel, dim3(D1 / 64, NF / 64), dim3(256), 0, st, n->p16 + OW1, NF, D1, n->p16t + T1);
+    hipLaunchKernelGGL(transpose_kernel, dim3(D1 / 64, D1 / 64), dim3(256), 0, st, n->p16 + OW2, D1, D1, n->p16t + T2);
+    hipLaunchKernelGGL(transpose_kernel, dim3(D1 / 64, D1 / 64), dim3(256), 0, st, n->p16 + OW3, D1, D1, n->p16t + T3);
+}
+
 static void mlp_free_acts(lkl_mlp* n) {
-    void* b[] = {n->h1, n->h2, n->h3, n->dza, n->dzb, n->y, n->dy, n->dz32, n->part};
+    void* b[] = {n->h1, n->h2, n->h3, n->dza, n->dzb, n->y, n->dy, n->dz32, n->part, n->xt, n->ht, n->dzt};
     for (void* x : b)
         if (x) (void)hipFree(x);
     n->h1 = n->h2 = n->h3 = n->dza = n->dzb = nullptr;
+    n->xt = n->ht = n->dzt = nullptr;
     n->y = n->dy = n->dz32 = n->part = nullptr;
     n->cap = 0;
 }
@@ -795,7 +867,9 @@ static int mlp_reserve(lkl_mlp* n, size_t B) {
         hipMalloc((void**)&n->dzb, h * 2) ||
         hipMalloc((void**)&n->y, B * 4) || hipMalloc((void**)&n->dy, B * 4) ||
         hipMalloc((void**)&n->dz32, h * 4) ||
-        hipMalloc((void**)&n->part, ((B + CS_ROWS - 1) / CS_ROWS) * (size_t)D1 * 4)) {
+        hipMalloc((void**)&n->part, ((B + CS_ROWS - 1) / CS_ROWS) * (size_t)D1 * 4) ||
+        hipMalloc((void**)&n->xt, B * (size_t)NF * 2) || hipMalloc((void**)&n->ht, h * 2) ||
+        hipMalloc((void**)&n->dzt, h * 2)) {
         mlp_free_acts(n);
         serr("lkl_mlp: activation allocation failed");
         return -1;
@@ -809,7 +883,7 @@ LKL_API long long lkl_mlp_param_count(void) { return NPARAM; }
 LKL_API void lkl_mlp_destroy(lkl_mlp* n) {
     if (!n) return;
     mlp_free_acts(n);
-    void* b[] = {n->p, n->g, n->m, n->v, n->p16, n->loss};
+    void* b[] = {n->p, n->g, n->m, n->v, n->p16, n->p16t, n->loss};
     for (void* x : b)
         if (x) (void)hipFree(x);
     if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -843,7 +917,8 @@ LKL_API lkl_mlp* lkl_mlp_create(unsigned long long seed) {
     if (hipGetDevice(&n->device) != hipSuccess || hipStreamCreateWithFlags(&n->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void**)&n->p, NPARAM * 4) || hipMalloc((void**)&n->g, NPARAM * 4) ||
         hipMalloc((void**)&n->m, NPARAM * 4) || hipMalloc((void**)&n->v, NPARAM * 4) ||
-        hipMalloc((void**)&n->p16, NPARAM * 2) || hipMalloc((void**)&n->loss, 4) ||
+        hipMalloc((void**)&n->p16, NPARAM * 2) || hipMalloc((void**)&n->p16t, NT_W * 2) ||
+        hipMalloc((void**)&n->loss, 4) ||
         hipMemcpy(n->p, p.data(), NPARAM * 4, hipMemcpyHostToDevice) ||
         hipMemset(n->m, 0, NPARAM * 4) || hipMemset(n->v, 0, NPARAM * 4) || hipMemset(n->g, 0, NPARAM * 4)) {
         serr("lkl_mlp_create: device setup failed");
@@ -851,6 +926,7 @@ LKL_API lkl_mlp* lkl_mlp_create(unsigned long long seed) {
         return nullptr;
     }
     hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, n->stream, n->p, n->p16, NPARAM);
+    refresh_transposed(n, n->stream);
     if (hipStreamSynchronize(n->stream) != hipSuccess) {
         serr("lkl_mlp_create: init failed");
         lkl_mlp_destroy(n);
@@ -872,6 +948,7 @@ LKL_API int lkl_mlp_set_params(lkl_mlp* n, const float* in) {
     if (hipStreamSynchronize(n->stream) != hipSuccess) return -1;
     if (hipMemcpy(n->p, in, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
     hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, n->stream, n->p, n->p16, NPARAM);
+    refresh_transposed(n, n->stream);
     return hipStreamSynchronize(n->stream) == hipSuccess ? 0 : -1;
 }
 
@@ -886,7 +963,7 @@ static void gemm(hipStream_t st, const bf16* A, long sam, long sak, const bf16* 
                  int M, int N, int K, float* C32, bf16* C16, long ldc, const float* bias,
                  const bf16* mask) {
     GemmArgs a = {A, B, sam, sak, sbk, sbn, M, N, K, C32, C16, ldc, bias, mask};
-    hipLaunchKernelGGL(gemm_kernel<EPI>, dim3(N / TN, M / TM), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(gemm_kernel<EPI>, dim3((unsigned)((N / TN) * (M / TM))), dim3(256), 0, st, a);
 }
 
 static int check_batch(int B) {
@@ -900,9 +977,10 @@ LKL_API int lkl_mlp_forward(lkl_mlp* n, const void* d_x, int B, float* d_y, void
     hipStream_t st = stream ? (hipStream_t)stream : n->stream;
     const bf16* X = (const bf16*)d_x;
     const bf16* W = n->p16;
-    gemm<EPI_BIAS_RELU>(st, X, NF, 1, W + OW1, D1, 1, B, D1, NF, nullptr, n->h1, D1, n->p + OB1, nullptr);
-    gemm<EPI_BIAS_RELU>(st, n->h1, D1, 1, W + OW2, D1, 1, B, D1, D1, nullptr, n->h2, D1, n->p + OB2, nullptr);
-    gemm<EPI_BIAS_RELU>(st, n->h2, D1, 1, W + OW3, D1, 1, B, D1, D1, nullptr, n->h3, D1, n->p + OB3, nullptr);
+    const bf16* Wt = n->p16t;  // B(k, n) = Wt[n][k]: both operands K-contiguous
+    gemm<EPI_BIAS_RELU>(st, X, NF, 1, Wt + T1, 1, NF, B, D1, NF, nullptr, n->h1, D1, n->p + OB1, nullptr);
+    gemm<EPI_BIAS_RELU>(st, n->h1, D1, 1, Wt + T2, 1, D1, B, D1, D1, nullptr, n->h2, D1, n->p + OB2, nullptr);
+    gemm<EPI_BIAS_RELU>(st, n->h2, D1, 1, Wt + T3, 1, D1, B, D1, D1, nullptr, n->h3, D1, n->p + OB3, nullptr);
     hipLaunchKernelGGL(head_forward, dim3((B + 3) / 4), dim3(256), 0, st, n->h3, W + OW4, n->p + OB4, B, d_y);
     if (hipGetLastError() != hipSuccess) { serr("lkl_mlp_forward: launch failed"); return -1; }
     return 0;
@@ -931,16 +1009,30 @@ LKL_API int lkl_mlp_grad(lkl_mlp* n, const void* d_x, const float* d_t, int B, f
                        n->dz32, n->dza);
     colsum(st, nullptr, n->h3, n->dy, B, D1, n->part, G + OW4);  // dW4 = H3^T dy
     colsum(st, n->dy, nullptr, nullptr, B, 1, n->part, G + OB4);  // db4 = sum dy
+    // Weight gradients dW = H^T dZ run on transposed copies (H^T and dZ^T, batch
+    // contiguous), so both GEMM operands are K-contiguous.
+    auto tr = [&](const bf16* src, int rows, int cols, bf16* dst) {
+        hipLaunchKernelGGL(transpose_kernel, dim3(cols / 64, rows / 64), dim3(256), 0, st, src, rows, cols, dst);
+    };
+    auto dw = [&](const bf16* Ht, int nin, float* out) {
+        gemm<EPI_F32>(st, Ht, B, 1, n->dzt, 1, B, nin, D1, B, out, nullptr, D1, nullptr, nullptr);
+    };
     // layer 3: dW3 = H2^T dZ3, db3, dZ2 = dZ3 W3^T (H2 > 0)
-    gemm<EPI_F32>(st, n->h2, 1, D1, n->dza, D1, 1, D1, D1, B, G + OW3, nullptr, D1, nullptr, nullptr);
+    tr(n->h2, B, D1, n->ht);
+    tr(n->dza, B, D1, n->dzt);
+    dw(n->ht, D1, G + OW3);
     colsum(st, n->dz32, nullptr, nullptr, B, D1, n->part, G + OB3);
     gemm<EPI_MASK>(st, n->dza, D1, 1, W + OW3, 1, D1, B, D1, D1, n->dz32, n->dzb, D1, nullptr, n->h2);
     // layer 2: dW2 = H1^T dZ2, db2, dZ1 = dZ2 W2^T (H1 > 0)
-    gemm<EPI_F32>(st, n->h1, 1, D1, n->dzb, D1, 1, D1, D1, B, G + OW2, nullptr, D1, nullptr, nullptr);
+    tr(n->h1, B, D1, n->ht);
+    tr(n->dzb, B, D1, n->dzt);
+    dw(n->ht, D1, G + OW2);
     colsum(st, n->dz32, nullptr, nullptr, B, D1, n->part, G + OB2);
     gemm<EPI_MASK>(st, n->dzb, D1, 1, W + OW2, 1, D1, B, D1, D1, n->dz32, n->dza, D1, nullptr, n->h1);
     // layer 1: dW1 = X^T dZ1, db1
-    gemm<EPI_F32>(st, X, 1, NF, n->dza, D1, 1, NF, D1, B, G + OW1, nullptr, D1, nullptr, nullptr);
+    tr(X, B, NF, n->xt);
+    tr(n->dza, B, D1, n->dzt);
+    dw(n->xt, NF, G + OW1);
     colsum(st, n->dz32, nullptr, nullptr, B, D1, n->part, G + OB1);
     if (hipGetLastError() != hipSuccess) { serr("lkl_mlp_grad: launch failed"); return -1; }
     if (loss_out) {
@@ -957,6 +1049,7 @@ LKL_API int lkl_mlp_adam(lkl_mlp* n, float lr, float beta1, float beta2, float e
     const float c1 = 1.0f - std::pow(beta1, (float)n->step), c2 = 1.0f - std::pow(beta2, (float)n->step);
     hipLaunchKernelGGL(adam_kernel, dim3(2048), dim3(256), 0, n->stream, n->p, n->g, n->m, n->v, n->p16,
                        NPARAM, lr, beta1, beta2, eps, c1, c2);
+    refresh_transposed(n, n->stream);
     if (hipGetLastError() != hipSuccess) { serr("lkl_mlp_adam: launch failed"); return -1; }
     return 0;
 }

@@ -21,7 +21,7 @@ def test_gemm_bf16_layouts(gpu, layout):
     lane layout fails the f64 comparison.  Bound: fp32 accumulation error over
     K products, 2^-22 * K * sum|a||b| (loose) — observed errors are far below."""
     rng = np.random.default_rng(7)
-    Mm, N, K = 256, 384, 96
+    Mm, N, K = 256, 384, 192
     A = rng.normal(size=(Mm, K))
     B = rng.normal(size=(K, N))
     Ab, Bb = _bits(A), _bits(B)
@@ -43,11 +43,11 @@ def test_gemm_bf16_layouts(gpu, layout):
 
 @pytest.mark.gpu
 def test_gemm_rejects_bad_shapes(gpu):
-    a = np.zeros(128 * 32, np.uint16)
+    a = np.zeros(128 * 64, np.uint16)
     with pytest.raises(Exception):
-        S.gemm_bf16(a, 32, 1, a, 128, 1, 128, 128, 40)  # K not a multiple of 32
+        S.gemm_bf16(a, 32, 1, a, 128, 1, 128, 128, 40)  # K not a multiple of 64
     with pytest.raises(Exception):
-        S.gemm_bf16(a, 32, 1, a[:100], 128, 1, 128, 128, 32)  # B too short
+        S.gemm_bf16(a, 64, 1, a[:100], 128, 1, 128, 128, 64)  # B too short
 
 
 @pytest.fixture(scope="module")
