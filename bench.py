@@ -84,6 +84,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=-1)
     ap.add_argument("--cell-x", type=float, default=0.0, help="receptor cell edge along x (0 = cubic)")
+    ap.add_argument("--list-h", type=float, default=-1.0,
+                    help="neighbour-list node spacing in A (0 = cell-row walk; default: the library's 1 A)")
     ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
     ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
                     help="c2: 12.6k-atom metric workload; c3: 8k x 3k cpydock, 400 poses/step; "
@@ -149,6 +151,8 @@ def main():
 
     K, W, N = args.steps, args.warmup, (400 if args.workload == "c3" else POSES_PER_STEP)
     poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
+    if args.list_h >= 0:
+        ctx.set_list_spacing(args.list_h)
     tune = None
     if args.cell or args.block or args.chunk >= 0 or args.cell_x:
         tiles = (args.cell or 5.0, args.block or 256, max(args.chunk, 0), args.cell_x or (args.cell or 5.0))
@@ -164,6 +168,9 @@ def main():
             % (cell, cell_x, block, chunk, tune["seconds"]))
     else:
         tiles = (5.0, 256, 0, 5.0)
+    n_list, list_dims = ctx.list_info()
+    log("near field: %s" % ("neighbour lists, %d entries over %s nodes" % (n_list, list_dims) if n_list
+                            else "cell-row walk"))
     d_poses = torch.from_numpy(poses).to(dev)
     d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
     d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
@@ -240,6 +247,9 @@ def main():
                    "family": args.family, "flags": flags, "poses_per_step": N,
                    "cell_edge_A": tiles[0], "cell_edge_x_A": tiles[3], "block": tiles[1],
                    "ligand_atoms_per_tile": tiles[2],
+                   "near_field": ({"neighbour_lists": {"spacing_A": args.list_h if args.list_h > 0 else 1.0,
+                                                       "entries": n_list, "nodes": list(list_dims)}}
+                                  if n_list else "cell-row walk"),
                    "autotune": tune},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None,

@@ -167,6 +167,21 @@ int lkl_ctx_set_cell_x(lkl_ctx* ctx, float cell_x);
  * identical either way. */
 int lkl_ctx_set_work_sort(lkl_ctx* ctx, int on);
 
+/* Neighbour lists of the near field (default on, spacing 1 A): for each node
+ * of a receptor grid of that spacing, the receptor atoms within the 10.025 A
+ * window plus the node's half-diagonal, built on the device once per receptor
+ * layout.  A ligand atom then scans its nearest node's list instead of walking
+ * cell rows.  spacing 0 switches them off (cell-row walk); they are also off
+ * for receptors above 65,535 atoms or when the tables would exceed 2^30
+ * entries.  Scores are bitwise identical either way.  Library-internal layout
+ * with no counterpart in the reference (cells are built by
+ * src/gpu_score.rs:61-112 there). */
+int lkl_ctx_set_list_spacing(lkl_ctx* ctx, float spacing);
+
+/* Neighbour-list state: returns the number of list entries (0 = lists off,
+ * -1 = null context) and the node grid dimensions in dims3 (may be NULL). */
+long long lkl_ctx_list_info(const lkl_ctx* ctx, int* dims3);
+
 /* Autotuner: time every (cell edge y/z, cell edge x, workgroup size, ligand
  * tile) candidate on N host poses with `flags`, keep the fastest in the context
  * and report it.  Scores are bitwise identical for every candidate.  table

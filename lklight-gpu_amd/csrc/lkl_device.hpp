@@ -38,6 +38,8 @@ constexpr float SOLV_HI = 40.97f;      // f32 filter of the desolvation window
 constexpr float WIN2 = 100.5f;         // cell window radius^2 (10.025 A, > 10 A + rounding)
 constexpr float SOLV_KEY_T = 40.96f;   // largest float <= SOLV2
 constexpr double MIN_INIT = 10000.0;   // HUGE_DISTANCE of cpydock.rs:592
+constexpr int LIST_PAD = 4;            // neighbour-list entries per aligned 8-byte load
+constexpr float DUMMY_XYZ = 1.0e15f;   // coordinate of the padding record (d2 ~ 3e30, finite)
 
 // Family flag word (full_score.cu:23-25) + desolvation bit.
 constexpr unsigned F_FAR = 1u, F_ELEC = 2u, F_CLASH = 4u, F_DESOLV = 16u;
@@ -60,6 +62,15 @@ struct DevComplex {
     const unsigned short* wgrid;
     int wnx, wny, wnz;
     float wox, woy, woz, inv_wh;
+    // neighbour lists (NULL = cell-row walk): for each node of a grid of
+    // spacing 1/inv_nh at no + i/inv_nh, the sorted receptor indices of the
+    // atoms within rwin + (node half-diagonal) of the node, in sorted order,
+    // each list padded to a multiple of LIST_PAD with index nr (a dummy record
+    // far from everything).  noff holds nodes + 1 offsets into nidx.
+    const unsigned* noff;
+    const unsigned short* nidx;
+    int nnx, nny, nnz;
+    float nox, noy, noz, inv_nh;
     // ligand, spatially sorted (Morton order of the reference coordinates)
     const float4* lxyzq;
     const float4* lpar;

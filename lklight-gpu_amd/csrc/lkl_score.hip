@@ -78,9 +78,6 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 #ifndef LKL_NU
 #define LKL_NU 4
 #endif
-#ifndef LKL_PRE_RPAR
-#define LKL_PRE_RPAR 0
-#endif
 #ifndef SQRT_W
 #define SQRT_W(x) __builtin_amdgcn_sqrtf(x)
 #endif
@@ -89,6 +86,55 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 #endif
 constexpr int NU = LKL_NU;
 
+// One receptor candidate r (sorted index p) against ligand atom (x64, y64,
+// z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
+// directly in fixed-point units, and (with desolvation) the per-atom minima of
+// cpydock.rs:620-627.  (xh + xl) is the hi/lo f32 split of the f64 coordinate.
+template <bool ELEC, bool CLASH, int DES>
+__device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, int p, double x64, double y64,
+                                          double z64, float xh, float yh, float zh, float xl, float yl,
+                                          float zl, float qjs, float svjs, float rj, bool hj, bool lf,
+                                          long long& e, long long& v, double& minL,
+                                          unsigned* __restrict__ minR) {
+    const float dx = (xh - r.x) + xl;
+    const float dy = (yh - r.y) + yl;
+    const float dz = (zh - r.z) + zl;
+    const float d2 = dx * dx + dy * dy + dz * dz;
+    if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
+        if (C.rflag[p] == 0) {
+            const double d2e = exact_d2(x64, y64, z64, r);
+            minL = fmin(minL, d2e);
+            atomicMin(&minR[p], solv_key(d2e));
+        }
+    }
+    if (d2 <= NEAR_HI && !(d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2)) {
+        const float4 rp = C.rpar[p];
+        const float inv = __builtin_amdgcn_rcpf(d2);
+        if (ELEC) {  // clamp to +-ES_CAP (selects, no branches)
+            const float ae = qjs * r.w * inv;
+            int te = __float2int_rn(ae);
+            te = ae > ES_CAP_SF ? (int)ES_CAP_I : te;
+            te = ae < -ES_CAP_SF ? -(int)ES_CAP_I : te;
+            e += te;
+        }
+        const float svs = svjs * rp.x;
+        const float rr = rj + rp.y;
+        const float rr2 = rr * rr;
+        const float i2 = inv * inv;
+        const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
+        float vp = svs * (p6 * p6 - 2.0f * p6);
+        vp = fminf(vp, LJ_CAP * V_SCALE_F);
+        if (CLASH && hj && rp.z != 0.0f) {
+            const float d = SQRT_D(d2);
+            const float dmin = CP_F * rr;
+            if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+        }
+        v += (long long)__float2int_rn(vp);
+    }
+}
+
+// Near field of one ligand atom by the cell-row walk: scan the receptor cell
+// rows whose cells meet the 10.025 A ball.
 template <bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, double y64, double z64,
                                           float qj, float svj, float rj, bool hj, bool lf,
@@ -129,67 +175,77 @@ __device__ __forceinline__ void near_atom(const DevComplex& C, double x64, doubl
             // the distance tests then run on data already in flight.
             for (int p0 = pb; p0 < pe; p0 += NU) {
                 float4 rq[NU];
-#if LKL_PRE_RPAR
-                float4 rpq[NU];
-#endif
+#pragma unroll
+                for (int k = 0; k < NU; ++k) rq[k] = C.rxyzq[min(p0 + k, pe - 1)];
 #pragma unroll
                 for (int k = 0; k < NU; ++k) {
-                    const int pk = min(p0 + k, pe - 1);
-                    rq[k] = C.rxyzq[pk];
-#if LKL_PRE_RPAR
-                    rpq[k] = C.rpar[pk];
-#endif
-                }
-#pragma unroll
-                for (int k = 0; k < NU; ++k) {
-                    const int p = p0 + k;
-                    if (p >= pe) break;
-                    const float4 r = rq[k];
-                    const float dx = (xh - r.x) + xl;
-                    const float dy = (yh - r.y) + yl;
-                    const float dz = (zh - r.z) + zl;
-                    const float d2 = dx * dx + dy * dy + dz * dz;
-                    if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
-                        if (C.rflag[p] == 0) {
-                            const double d2e = exact_d2(x64, y64, z64, r);
-                            minL = fmin(minL, d2e);
-                            atomicMin(&minR[p], solv_key(d2e));
-                        }
-                    }
-                    if (d2 <= NEAR_HI && !(d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2)) {
-#if LKL_PRE_RPAR
-                        const float4 rp = rpq[k];
-#else
-                        const float4 rp = C.rpar[p];
-#endif
-                        const float inv = __builtin_amdgcn_rcpf(d2);
-                        if (ELEC) {  // clamp to +-ES_CAP (selects, no branches)
-                            const float ae = qjs * r.w * inv;
-                            int te = __float2int_rn(ae);
-                            te = ae > ES_CAP_SF ? (int)ES_CAP_I : te;
-                            te = ae < -ES_CAP_SF ? -(int)ES_CAP_I : te;
-                            e += te;
-                        }
-                        const float svs = svjs * rp.x;
-                        const float rr = rj + rp.y;
-                        const float rr2 = rr * rr;
-                        const float i2 = inv * inv;
-                        const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
-                        float vp = svs * (p6 * p6 - 2.0f * p6);
-                        vp = fminf(vp, LJ_CAP * V_SCALE_F);
-                        if (CLASH && hj && rp.z != 0.0f) {
-                            const float d = SQRT_D(d2);
-                            const float dmin = CP_F * rr;
-                            if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
-                        }
-                        v += (long long)__float2int_rn(vp);
-                    }
+                    if (p0 + k >= pe) break;
+                    pair_term<ELEC, CLASH, DES>(C, rq[k], p0 + k, x64, y64, z64, xh, yh, zh, xl, yl, zl, qjs,
+                                                svjs, rj, hj, lf, e, v, minL, minR);
                 }
             }
         }
     }
 }
 
+// Node of the neighbour-list grid nearest to (x, y, z), or -1 outside the grid
+// (then no receptor atom is within reach).  NaN coordinates give -1.
+__device__ __forceinline__ int list_node(const DevComplex& C, double x, double y, double z) {
+    const float fx = ((float)x - C.nox) * C.inv_nh + 0.5f;
+    const float fy = ((float)y - C.noy) * C.inv_nh + 0.5f;
+    const float fz = ((float)z - C.noz) * C.inv_nh + 0.5f;
+    if (!(fx >= 0.0f && fx < (float)C.nnx && fy >= 0.0f && fy < (float)C.nny && fz >= 0.0f &&
+          fz < (float)C.nnz))
+        return -1;
+    return ((int)fz * C.nny + (int)fy) * C.nnx + (int)fx;
+}
+
+// Near field of one ligand atom by its node's neighbour list [b, e): one
+// aligned 8-byte load gives LIST_PAD receptor indices, whose records are then
+// requested together; padding entries point at the dummy record, which fails
+// every distance test.
+template <bool ELEC, bool CLASH, int DES>
+__device__ __forceinline__ void near_list(const DevComplex& C, double x64, double y64, double z64,
+                                          float qj, float svj, float rj, bool hj, bool lf, unsigned b,
+                                          unsigned en, long long& e, long long& v, double& minL,
+                                          unsigned* __restrict__ minR) {
+    static_assert(LIST_PAD == 4, "one ushort4 per step");
+    const float qjs = qj * E_SCALE_F;
+    const float svjs = svj * V_SCALE_F;
+    const float xh = (float)x64, yh = (float)y64, zh = (float)z64;
+    const float xl = (float)(x64 - (double)xh);
+    const float yl = (float)(y64 - (double)yh);
+    const float zl = (float)(z64 - (double)zh);
+    for (unsigned p0 = b; p0 < en; p0 += LIST_PAD) {
+        const ushort4 q = *reinterpret_cast<const ushort4*>(C.nidx + p0);
+        const int id[4] = {(int)q.x, (int)q.y, (int)q.z, (int)q.w};
+        float4 rq[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rq[k] = C.rxyzq[id[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            pair_term<ELEC, CLASH, DES>(C, rq[k], id[k], x64, y64, z64, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
+                                        hj, lf, e, v, minL, minR);
+    }
+}
+
+// Near field of one ligand atom: neighbour lists (LIST; the context has them)
+// or the cell-row walk.  Both visit a superset of the pairs within 10 A and
+// apply the same tests, so the fixed-point sums are identical.  LIST is a
+// template parameter so each kernel carries one walk (fewer registers).
+template <bool LIST, bool ELEC, bool CLASH, int DES>
+__device__ __forceinline__ void near_any(const DevComplex& C, double x64, double y64, double z64, float qj,
+                                         float svj, float rj, bool hj, bool lf, long long& e, long long& v,
+                                         double& minL, unsigned* __restrict__ minR) {
+    if constexpr (LIST) {
+        const int n = list_node(C, x64, y64, z64);
+        if (n >= 0)
+            near_list<ELEC, CLASH, DES>(C, x64, y64, z64, qj, svj, rj, hj, lf, C.noff[n], C.noff[n + 1], e, v,
+                                        minL, minR);
+    } else {
+        near_atom<ELEC, CLASH, DES>(C, x64, y64, z64, qj, svj, rj, hj, lf, e, v, minL, minR);
+    }
+}
 
 // ── work grid ────────────────────────────────────────────────────────────────
 // Per node (spacing WORK_H over the receptor box grown by WORK_PAD): the number
@@ -217,6 +273,98 @@ __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox
     out[node] = (unsigned short)min(cnt, 65535u);
 }
 
+// ── neighbour lists ──────────────────────────────────────────────────────────
+// One workgroup owns an 8x8x4 brick of list nodes (one lane per node).  It
+// streams the sorted receptor in index order, keeps the atoms within reach of
+// the brick's box (order-preserving compaction into LDS) and every lane then
+// tests them against its node: |node - r|^2 <= rad2 in f32 (rad carries a
+// 0.01 A margin over the window radius plus the half-diagonal).  COUNT writes
+// each node's list length rounded up to LIST_PAD; FILL writes the indices at
+// the node's offset, in receptor order, then the padding index nr.  Both
+// instantiations evaluate the same test, so they agree on every list.
+constexpr int NLX = 8, NLY = 8, NLZ = 4, NLT = NLX * NLY * NLZ;
+constexpr int NLCAP = 2048;  // LDS candidate buffer (32 KB)
+
+template <bool FILL>
+__global__ void __launch_bounds__(NLT)
+nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz, float h, int nx, int ny,
+             int nz, float rad, unsigned* __restrict__ cnt_off, unsigned short* __restrict__ idx) {
+#pragma clang fp contract(off)
+    __shared__ float4 buf[NLCAP];
+    __shared__ int wave_cnt[NLT / 64];
+    __shared__ int count;
+    const int tid = threadIdx.x;
+    const int ix = blockIdx.x * NLX + (tid % NLX);
+    const int iy = blockIdx.y * NLY + ((tid / NLX) % NLY);
+    const int iz = blockIdx.z * NLZ + (tid / (NLX * NLY));
+    const bool valid = ix < nx && iy < ny && iz < nz;
+    const size_t node = valid ? ((size_t)iz * ny + iy) * nx + ix : 0;
+    const float px = ox + (float)ix * h, py = oy + (float)iy * h, pz = oz + (float)iz * h;
+    // brick box over its node positions (the same f32 expressions, monotone in i)
+    const int bx = blockIdx.x * NLX, by = blockIdx.y * NLY, bz = blockIdx.z * NLZ;
+    const float bx0 = ox + (float)bx * h, bx1 = ox + (float)(bx + NLX - 1) * h;
+    const float by0 = oy + (float)by * h, by1 = oy + (float)(by + NLY - 1) * h;
+    const float bz0 = oz + (float)bz * h, bz1 = oz + (float)(bz + NLZ - 1) * h;
+    const float rad2 = rad * rad, pre2 = (rad + 0.01f) * (rad + 0.01f);
+    unsigned k = 0, base_out = 0, end_out = 0;
+    if (FILL && valid) {
+        base_out = cnt_off[node];
+        end_out = cnt_off[node + 1];  // the COUNT pass's padded length bounds every write
+    }
+    if (tid == 0) count = 0;
+    __syncthreads();
+    for (int base = 0; base < nr; base += NLT) {
+        const int i = base + tid;
+        bool take = false;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (i < nr) {
+            a = rec[i];
+            const float gx = fmaxf(0.f, fmaxf(bx0 - a.x, a.x - bx1));
+            const float gy = fmaxf(0.f, fmaxf(by0 - a.y, a.y - by1));
+            const float gz = fmaxf(0.f, fmaxf(bz0 - a.z, a.z - bz1));
+            take = gx * gx + gy * gy + gz * gz <= pre2;
+        }
+        const unsigned long long m = __ballot(take);
+        const int lane = tid & 63, w = tid >> 6;
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) wave_cnt[w] = __popcll(m);
+        __syncthreads();
+        int off = count;
+        for (int q = 0; q < w; ++q) off += wave_cnt[q];
+        if (take) buf[off + before] = make_float4(a.x, a.y, a.z, __int_as_float(i));
+        __syncthreads();
+        if (tid == 0) {
+            int t = count;
+            for (int q = 0; q < NLT / 64; ++q) t += wave_cnt[q];
+            count = t;
+        }
+        __syncthreads();
+        const int n = count;
+        if (n > NLCAP - NLT || base + NLT >= nr) {
+            if (valid) {
+                for (int q = 0; q < n; ++q) {
+                    const float4 b = buf[q];
+                    const float dx = px - b.x, dy = py - b.y, dz = pz - b.z;
+                    if (dx * dx + dy * dy + dz * dz <= rad2) {
+                        if (FILL && base_out + k < end_out) idx[base_out + k] = (unsigned short)__float_as_int(b.w);
+                        ++k;
+                    }
+                }
+            }
+            __syncthreads();
+            if (tid == 0) count = 0;
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+    const unsigned padded = (k + (unsigned)LIST_PAD - 1u) / (unsigned)LIST_PAD * (unsigned)LIST_PAD;
+    if (FILL) {
+        for (; base_out + k < end_out; ++k) idx[base_out + k] = (unsigned short)nr;
+    } else {
+        cnt_off[node] = padded;
+    }
+}
+
 // Bucket of a work count: NBKT (last) for no work, else descending
 // quarter-octaves of the count (larger counts first).
 #ifndef LKL_NBKT
@@ -238,6 +386,17 @@ __device__ __forceinline__ unsigned work_at(const DevComplex& C, double x, doubl
     return C.wgrid[((size_t)iz * C.wny + iy) * C.wnx + ix];
 }
 
+// Expected near-field work of a ligand atom: its neighbour-list length when
+// the context has lists (exact), else the work-grid count.  0 = nothing in reach.
+template <bool LIST>
+__device__ __forceinline__ unsigned work_of(const DevComplex& C, double x, double y, double z) {
+    if constexpr (LIST) {
+        const int n = list_node(C, x, y, z);
+        return n < 0 ? 0u : C.noff[n + 1] - C.noff[n];
+    }
+    return work_at(C, x, y, z);
+}
+
 // Near field (+ far field) of ligand atoms [j0, j1) of one pose, work-ordered:
 // pass 1 transforms every atom, adds its far field and files it in a bucket
 // (descending work-grid count; NBKT = none); pass 2 walks the windows of the
@@ -245,7 +404,7 @@ __device__ __forceinline__ unsigned work_at(const DevComplex& C, double x, doubl
 // similar size.  With desolvation (whole-ligand tiles) the per-atom ligand term
 // is added in whichever pass finishes the atom.  All threads of the block must
 // call it (barriers); lds holds (j1 - j0) * 3 bytes.
-template <bool FAR, bool ELEC, bool CLASH, int DES>
+template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                             unsigned* __restrict__ minR, long long& e, long long& v,
                                             long long& s) {
@@ -261,7 +420,7 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
         double x, y, z;
         transform(M, lb.x, lb.y, lb.z, x, y, z);
         if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
-        const int bb = work_bucket(work_at(C, x, y, z));
+        const int bb = work_bucket(work_of<LIST>(C, x, y, z));
         bkt[t] = (unsigned char)bb;
         atomicAdd(&bcnt[bb], 1u);
         // no receptor atom in reach: the desolvation minimum stays at its initial value
@@ -288,7 +447,7 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
         transform(M, lb.x, lb.y, lb.z, x, y, z);
         double minL = MIN_INIT;
         const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
-        near_atom<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, e, v, minL, minR);
+        near_any<LIST, ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, e, v, minL, minR);
         if (DES != DES_NONE) s += to_fix(solv_term(minL, C.lasa[j], C.ldes[j]), E_SCALE);
     }
     __syncthreads();  // the LDS lists may be reused by the caller
@@ -303,7 +462,7 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 #ifndef LKL_SCORE_ATTR
 #define LKL_SCORE_ATTR
 #endif
-template <bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
+template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
 __global__ void __launch_bounds__(512) LKL_SCORE_ATTR
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
@@ -334,7 +493,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     if (!PERATOM && C.wgrid) {
         // lists after the receptor minima when those live in LDS
         void* lists = DES == DES_LDS ? (void*)(lds_min + C.nr) : (void*)lds_min;
-        near_sorted<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+        near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
     } else if constexpr (DES == DES_NONE && !PERATOM) {
         {
             for (int j = j0 + tid; j < j1; j += bs) {
@@ -344,7 +503,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
                 transform(M, lb.x, lb.y, lb.z, x, y, z);
                 if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
                 double minL = MIN_INIT;
-                near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
+                near_any<LIST, ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
                                                  minL, nullptr);
             }
         }
@@ -358,7 +517,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         double minL = MIN_INIT;
         if (FAR && lb.w != 0.0f) ea += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
         const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
-        near_atom<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, ea, va, minL,
+        near_any<LIST, ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, ea, va, minL,
                                     minR);
         if (PERATOM) {
             pa_e[(size_t)pose * C.nl + j] = (float)((double)ea / E_SCALE);
@@ -416,12 +575,12 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
 // ── multi-problem batch ───────────────────────────────────────────────────────
 // Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
 // thread's share (no desolvation, no per-atom output).
-template <bool FAR, bool ELEC, bool CLASH>
+template <bool LIST, bool FAR, bool ELEC, bool CLASH>
 __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
     if (C.wgrid) {
         long long s_unused = 0;
-        near_sorted<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+        near_sorted<LIST, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
     }
     for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
@@ -431,7 +590,7 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
         transform(M, lb.x, lb.y, lb.z, x, y, z);
         double minL = MIN_INIT;
         if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
-        near_atom<ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
+        near_any<LIST, ELEC, CLASH, DES_NONE>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, false, e, v,
                                          minL, nullptr);
     }
 }
@@ -453,6 +612,7 @@ struct ProblemDesc {
 // those of its own launch.  (No loop around barriers: a claim-loop variant
 // with a divergent tid==0 claim was restructured by the compiler into nested
 // loops that cycle the barriers without the claiming lane, which never ends.)
+template <bool LIST>
 __global__ void __launch_bounds__(512)
 multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__ probs, int K,
              int total_tiles, const double* __restrict__ poses, double* __restrict__ out,
@@ -477,14 +637,14 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
     long long e = 0, v = 0;
     switch (P.flags & 7u) {
-        case 0: tile_sums<false, false, false>(C, M, j0, j1, multi_lds, e, v); break;
-        case 1: tile_sums<true, false, false>(C, M, j0, j1, multi_lds, e, v); break;
-        case 2: tile_sums<false, true, false>(C, M, j0, j1, multi_lds, e, v); break;
-        case 3: tile_sums<true, true, false>(C, M, j0, j1, multi_lds, e, v); break;
-        case 4: tile_sums<false, false, true>(C, M, j0, j1, multi_lds, e, v); break;
-        case 5: tile_sums<true, false, true>(C, M, j0, j1, multi_lds, e, v); break;
-        case 6: tile_sums<false, true, true>(C, M, j0, j1, multi_lds, e, v); break;
-        default: tile_sums<true, true, true>(C, M, j0, j1, multi_lds, e, v); break;
+        case 0: tile_sums<LIST, false, false, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 1: tile_sums<LIST, true, false, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 2: tile_sums<LIST, false, true, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 3: tile_sums<LIST, true, true, false>(C, M, j0, j1, multi_lds, e, v); break;
+        case 4: tile_sums<LIST, false, false, true>(C, M, j0, j1, multi_lds, e, v); break;
+        case 5: tile_sums<LIST, true, false, true>(C, M, j0, j1, multi_lds, e, v); break;
+        case 6: tile_sums<LIST, false, true, true>(C, M, j0, j1, multi_lds, e, v); break;
+        default: tile_sums<LIST, true, true, true>(C, M, j0, j1, multi_lds, e, v); break;
     }
     e = wave_sum(e);
     v = wave_sum(v);
@@ -539,6 +699,12 @@ struct lkl_ctx {
     unsigned short* d_wgrid = nullptr;
     int wn[3] = {0, 0, 0};
     double wo[3] = {0, 0, 0};
+    float list_h = 1.0f;  // neighbour-list node spacing (A); 0 = cell-row walk
+    unsigned* d_noff = nullptr;
+    unsigned short* d_nidx = nullptr;
+    int nn[3] = {0, 0, 0};
+    float no[3] = {0.f, 0.f, 0.f};
+    size_t list_entries = 0;  // 0 = no lists (row walk)
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -603,6 +769,68 @@ static int dev_reserve(T** buf, size_t* cap, size_t n) {
 // atoms stably counting-sorted by cell (x fastest), packed float4 records.
 static float cell_x_of(const lkl_ctx* c) { return c->cell_x > 0.0f ? c->cell_x : c->cell_size; }
 
+// Neighbour lists over the sorted receptor (indices into d_rxyzq).  Lists are
+// skipped (row walk) when switched off, when an index would not fit 16 bits, or
+// when the node grid or the entry count exceeds the caps below.
+constexpr long long LIST_NODES_MAX = 1ll << 27;
+constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of indices
+
+static int build_lists(lkl_ctx* c) {
+    if (c->d_noff) { (void)hipFree(c->d_noff); c->d_noff = nullptr; }
+    if (c->d_nidx) { (void)hipFree(c->d_nidx); c->d_nidx = nullptr; }
+    c->list_entries = 0;
+    const int nr = c->nr;
+    if (!(c->list_h > 0.0f) || nr <= 0 || nr > 65535) return 0;
+    const float h = c->list_h;
+    const float rad = std::sqrt(WIN2) + 0.5f * std::sqrt(3.0f) * h + 0.01f;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < nr; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], (double)c->r_xyz[3 * i + a]);
+            hi[a] = std::max(hi[a], (double)c->r_xyz[3 * i + a]);
+        }
+    long long nodes = 1;
+    int n[3];
+    float o[3];
+    for (int a = 0; a < 3; ++a) {
+        o[a] = (float)std::floor(lo[a] - rad - h);
+        n[a] = (int)std::ceil((hi[a] + rad + h - (double)o[a]) / h) + 1;
+        nodes *= n[a];
+    }
+    if (nodes > LIST_NODES_MAX || (n[1] + NLY - 1) / NLY > 65535 || (n[2] + NLZ - 1) / NLZ > 65535) return 0;
+    HIPCK("alloc", hipMalloc((void**)&c->d_noff, sizeof(unsigned) * (size_t)(nodes + 1)));
+    dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
+    hipLaunchKernelGGL(nlist_kernel<false>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, nr, o[0], o[1], o[2], h,
+                       n[0], n[1], n[2], rad, c->d_noff, (unsigned short*)nullptr);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    std::vector<unsigned> off((size_t)nodes + 1);
+    HIPCK("readback", hipMemcpy(off.data(), c->d_noff, sizeof(unsigned) * (size_t)nodes, hipMemcpyDeviceToHost));
+    unsigned long long acc = 0;
+    for (long long k = 0; k < nodes; ++k) {
+        const unsigned cnt = off[k];
+        off[k] = (unsigned)acc;
+        acc += cnt;
+        if (acc > (unsigned long long)LIST_ENTRIES_MAX) break;
+    }
+    if (acc > (unsigned long long)LIST_ENTRIES_MAX) {
+        (void)hipFree(c->d_noff);
+        c->d_noff = nullptr;
+        return 0;
+    }
+    off[nodes] = (unsigned)acc;
+    HIPCK("upload", hipMemcpy(c->d_noff, off.data(), sizeof(unsigned) * off.size(), hipMemcpyHostToDevice));
+    HIPCK("alloc", hipMalloc((void**)&c->d_nidx, sizeof(unsigned short) * (size_t)std::max(acc, 4ull)));
+    hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, nr, o[0], o[1], o[2], h,
+                       n[0], n[1], n[2], rad, c->d_noff, c->d_nidx);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
+    c->list_entries = (size_t)std::max(acc, 1ull);
+    c->version++;
+    return 0;
+}
+
 static int build_receptor(lkl_ctx* c) {
     const int nr = c->nr;
     const double edge[3] = {(double)cell_x_of(c), (double)c->cell_size, (double)c->cell_size};
@@ -635,8 +863,12 @@ static int build_receptor(lkl_ctx* c) {
     for (size_t k = 0; k < ncell; ++k) start[k + 1] += start[k];
     std::vector<int> cur(start.begin(), start.end() - 1), order(nr);
     for (int i = 0; i < nr; ++i) order[cur[cell[i]]++] = i;
-    std::vector<float4> xyzq(nr), par(nr);
-    std::vector<unsigned char> flag(nr, 0);
+    // one extra record past the sorted atoms: the neighbour lists' padding
+    // entry, far from every atom (excluded from desolvation by its flag)
+    std::vector<float4> xyzq(nr + 1), par(nr + 1);
+    std::vector<unsigned char> flag(nr + 1, 1);
+    xyzq[nr] = make_float4(DUMMY_XYZ, DUMMY_XYZ, DUMMY_XYZ, 0.f);
+    par[nr] = make_float4(0.f, 0.f, 0.f, 0.f);
     std::vector<float> des(nr, 0.f), asa(nr, 0.f);
     for (int p = 0; p < nr; ++p) {
         const int i = order[p];
@@ -645,17 +877,18 @@ static int build_receptor(lkl_ctx* c) {
         par[p] = make_float4(c->r_svdw.empty() ? 0.f : c->r_svdw[i],
                              c->r_vdwr.empty() ? 0.f : c->r_vdwr[i],
                              (!c->r_heavy.empty() && c->r_heavy[i]) ? 1.f : 0.f, 0.f);
+        flag[p] = 0;
         if (c->has_desolv) {
             flag[p] = c->r_flag[i];
             des[p] = c->r_des[i];
             asa[p] = c->r_asa[i];
         }
     }
-    if (dev_upload(&c->d_rxyzq, xyzq.data(), (size_t)nr)) return -1;
-    if (dev_upload(&c->d_rpar, par.data(), (size_t)nr)) return -1;
+    if (dev_upload(&c->d_rxyzq, xyzq.data(), (size_t)nr + 1)) return -1;
+    if (dev_upload(&c->d_rpar, par.data(), (size_t)nr + 1)) return -1;
     if (dev_upload(&c->d_cstart, start.data(), ncell + 1)) return -1;
     if (c->has_desolv) {
-        if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr)) return -1;
+        if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr + 1)) return -1;
         if (dev_upload(&c->d_rdes, des.data(), (size_t)nr)) return -1;
         if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr)) return -1;
     }
@@ -691,7 +924,7 @@ static int build_receptor(lkl_ctx* c) {
     c->cox = lo[0];
     c->coy = lo[1];
     c->coz = lo[2];
-    return 0;
+    return build_lists(c);
 }
 
 // Morton code of a 10-bit-per-axis quantised point.
@@ -778,6 +1011,17 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.woy = (float)c->wo[1];
     C.woz = (float)c->wo[2];
     C.inv_wh = 1.0f / WORK_H;
+    if (c->list_entries) {
+        C.noff = c->d_noff;
+        C.nidx = c->d_nidx;
+        C.nnx = c->nn[0];
+        C.nny = c->nn[1];
+        C.nnz = c->nn[2];
+        C.nox = c->no[0];
+        C.noy = c->no[1];
+        C.noz = c->no[2];
+        C.inv_nh = 1.0f / c->list_h;
+    }
     C.lxyzq = c->d_lxyzq;
     C.lpar = c->d_lpar;
     C.lflag = c->d_lflag;
@@ -801,7 +1045,7 @@ static void ctx_free(lkl_ctx* c) {
     void* bufs[] = {c->d_phi, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
-                    c->d_acc, c->d_cnt, c->d_wgrid};
+                    c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -881,17 +1125,17 @@ static int reserve_acc(lkl_ctx* c, size_t N) {
     return 0;
 }
 
-template <int DES, bool PA>
+template <bool LIST, int DES, bool PA>
 static score_fn pick3(unsigned f) {
     switch (f & 7u) {
-        case 0: return score_kernel<false, false, false, DES, PA>;
-        case 1: return score_kernel<true, false, false, DES, PA>;
-        case 2: return score_kernel<false, true, false, DES, PA>;
-        case 3: return score_kernel<true, true, false, DES, PA>;
-        case 4: return score_kernel<false, false, true, DES, PA>;
-        case 5: return score_kernel<true, false, true, DES, PA>;
-        case 6: return score_kernel<false, true, true, DES, PA>;
-        default: return score_kernel<true, true, true, DES, PA>;
+        case 0: return score_kernel<LIST, false, false, false, DES, PA>;
+        case 1: return score_kernel<LIST, true, false, false, DES, PA>;
+        case 2: return score_kernel<LIST, false, true, false, DES, PA>;
+        case 3: return score_kernel<LIST, true, true, false, DES, PA>;
+        case 4: return score_kernel<LIST, false, false, true, DES, PA>;
+        case 5: return score_kernel<LIST, true, false, true, DES, PA>;
+        case 6: return score_kernel<LIST, false, true, true, DES, PA>;
+        default: return score_kernel<LIST, true, true, true, DES, PA>;
     }
 }
 
@@ -931,10 +1175,12 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     if (peratom && dev_reserve(&c->d_pa_e, &c->cap_pa, (size_t)N * c->nl)) return -1;
     if (peratom && dev_reserve(&c->d_pa_v, &c->cap_pa_v, (size_t)N * c->nl)) return -1;
     score_fn fn;
-    if (peratom) fn = pick3<DES_NONE, true>(flags);
-    else if (des == DES_LDS) fn = pick3<DES_LDS, false>(flags);
-    else if (des == DES_GLOBAL) fn = pick3<DES_GLOBAL, false>(flags);
-    else fn = pick3<DES_NONE, false>(flags);
+    const bool list = C.noff != nullptr;
+    if (peratom) fn = list ? pick3<true, DES_NONE, true>(flags) : pick3<false, DES_NONE, true>(flags);
+    else if (des == DES_LDS) fn = list ? pick3<true, DES_LDS, false>(flags) : pick3<false, DES_LDS, false>(flags);
+    else if (des == DES_GLOBAL)
+        fn = list ? pick3<true, DES_GLOBAL, false>(flags) : pick3<false, DES_GLOBAL, false>(flags);
+    else fn = list ? pick3<true, DES_NONE, false>(flags) : pick3<false, DES_NONE, false>(flags);
     int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
     if (des == DES_NONE && want_sort) {
         if (chunk > WORK_TILE_MAX) chunk = WORK_TILE_MAX;  // LDS list bound
@@ -1083,6 +1329,27 @@ LKL_API int lkl_ctx_set_work_sort(lkl_ctx* c, int on) {
     return 0;
 }
 
+LKL_API int lkl_ctx_set_list_spacing(lkl_ctx* c, float spacing) {
+    if (!c) { set_err("lkl_ctx_set_list_spacing: null context"); return -1; }
+    if (!(spacing == 0.0f || (spacing >= 0.25f && spacing <= 4.0f))) {
+        set_err("list spacing %.3f not 0 or in [0.25, 4]", spacing);
+        return -1;
+    }
+    HIPCK("probe", hipSetDevice(c->device));
+    if (spacing != c->list_h) {
+        c->list_h = spacing;
+        if (build_lists(c)) return -1;
+        c->version++;
+    }
+    return 0;
+}
+
+LKL_API long long lkl_ctx_list_info(const lkl_ctx* c, int* dims3) {
+    if (!c) return -1;
+    if (dims3) { dims3[0] = c->nn[0]; dims3[1] = c->nn[1]; dims3[2] = c->nn[2]; }
+    return c->list_entries ? (long long)c->list_entries : 0;
+}
+
 LKL_API int lkl_ctx_set_cell_x(lkl_ctx* c, float cell_x) {
     if (!c) { set_err("lkl_ctx_set_cell_x: null context"); return -1; }
     if (!(cell_x == 0.0f || (cell_x > 0.5f && cell_x <= 20.0f))) { set_err("cell_x %.3f not 0 or in (0.5, 20]", cell_x); return -1; }
@@ -1190,6 +1457,7 @@ struct lkl_multi {
     int K = 0, P = 0, total_tiles = 0, device = 0, cus = 256;
     int block = 256, chunk_mult = 2;
     size_t lds = 0;  // dynamic LDS of the work-ordered tiles (largest chunk x 3 bytes)
+    bool list = false;  // every problem has neighbour lists (or no receptor atoms)
     hipStream_t stream = nullptr;
     DevComplex* d_cx = nullptr;
     ProblemDesc* d_prob = nullptr;
@@ -1229,10 +1497,12 @@ static int batch_refresh(lkl_multi* b) {
     std::vector<ProblemDesc> pd(b->K);
     int tile = 0;
     b->lds = 0;
+    b->list = true;
     for (int r = 0; r < b->K; ++r) {
         const int k = b->order[r];
         lkl_ctx* c = b->ctxs[k];
         cx[r] = dev_view(c);
+        if (!cx[r].noff && c->nr > 0) b->list = false;
         int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
         if (cx[r].wgrid) {
             chunk = std::min(chunk, WORK_TILE_MAX);  // LDS list bound
@@ -1259,10 +1529,14 @@ static int batch_check(lkl_multi* b) {
     return 0;
 }
 
+typedef void (*multi_fn)(const DevComplex*, const ProblemDesc*, int, int, const double*, double*, long long*,
+                         unsigned*);
+
 static int batch_launch(lkl_multi* b, const double* d_poses, double* d_out, hipStream_t st) {
     if (b->total_tiles <= 0) return 0;
-    hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, st,
-                       b->d_cx, b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt);
+    multi_fn fn = b->list ? multi_kernel<true> : multi_kernel<false>;
+    hipLaunchKernelGGL(fn, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, st, b->d_cx,
+                       b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt);
     HIPCK("launch", hipGetLastError());
     return 0;
 }
@@ -1360,10 +1634,12 @@ LKL_API int lkl_multi_score(lkl_multi* b, const double* poses, double* out) {
         HIPCK("capture", hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e1 = hipMemcpyAsync(b->d_poses, b->h_poses, sizeof(double) * 7 * P,
                                        hipMemcpyHostToDevice, b->stream);
-        if (b->total_tiles > 0)
-            hipLaunchKernelGGL(multi_kernel, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds,
-                               b->stream, b->d_cx, b->d_prob, b->K, b->total_tiles, b->d_poses,
-                               b->d_out, b->d_acc, b->d_cnt);
+        if (b->total_tiles > 0) {
+            multi_fn fn = b->list ? multi_kernel<true> : multi_kernel<false>;
+            hipLaunchKernelGGL(fn, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, b->stream,
+                               b->d_cx, b->d_prob, b->K, b->total_tiles, b->d_poses, b->d_out, b->d_acc,
+                               b->d_cnt);
+        }
         hipError_t e2 = hipGetLastError();
         hipError_t e3 = hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 2 * P,
                                        hipMemcpyDeviceToHost, b->stream);

@@ -145,6 +145,16 @@ class Context:
         """lkl_ctx_set_work_sort: order each tile's near field by expected work."""
         _capi.check(lib().lkl_ctx_set_work_sort(self.h, int(bool(on))), "lkl_ctx_set_work_sort")
 
+    def set_list_spacing(self, spacing):
+        """lkl_ctx_set_list_spacing: neighbour-list node spacing (A); 0 = cell-row walk."""
+        _capi.check(lib().lkl_ctx_set_list_spacing(self.h, float(spacing)), "lkl_ctx_set_list_spacing")
+
+    def list_info(self):
+        """(list entries, node grid dims); entries 0 = lists off."""
+        d = np.zeros(3, np.int32)
+        n = lib().lkl_ctx_list_info(self.h, _capi.ptr(d, C.c_int))
+        return int(n), tuple(int(v) for v in d)
+
     def set_tiles(self, cell_size, block, chunk=0, cell_x=0.0):
         """Cell edge (y, z; and x unless cell_x > 0), workgroup size, ligand atoms per tile."""
         _capi.check(lib().lkl_ctx_set_cell_x(self.h, float(cell_x)), "lkl_ctx_set_cell_x")

@@ -116,17 +116,24 @@ def test_cpydock_parity(small):
 
 def test_tile_and_order_invariance(small):
     """Fixed-point per-pose sums: bitwise identical for every cell edge (cubic
-    and narrower along x), block size, ligand tile, and input atom order."""
+    and narrower along x), block size, ligand tile, neighbour-list spacing (or
+    the cell-row walk), and input atom order."""
     cx, orc, poses = small
     say("tile invariance")
     ctx = lklight.Context(cx, desolv=True)
     flags = lklight.FLAGS_DNA
+    ctx.set_list_spacing(0.0)  # reference run: cell-row walk
+    assert ctx.list_info()[0] == 0
     base = run(ctx, poses, flags)
     base_c = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
-    for cell, block, chunk, cell_x in [(10.0, 256, 0, 0.0), (3.4, 64, 64, 0.0), (6.0, 512, 0, 0.0),
-                                       (4.0, 128, 256, 0.0), (5.0, 64, 128, 0.0), (5.0, 256, 0, 1.25),
-                                       (7.5, 128, 64, 2.5), (3.4, 512, 0, 0.75)]:
+    for cell, block, chunk, cell_x, lh in [(10.0, 256, 0, 0.0, 1.0), (3.4, 64, 64, 0.0, 0.0),
+                                           (6.0, 512, 0, 0.0, 0.5), (4.0, 128, 256, 0.0, 2.0),
+                                           (5.0, 64, 128, 0.0, 1.5), (5.0, 256, 0, 1.25, 0.0),
+                                           (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0)]:
+        say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh}")
         ctx.set_tiles(cell, block, chunk, cell_x=cell_x)
+        ctx.set_list_spacing(lh)
+        assert (ctx.list_info()[0] > 0) == (lh > 0)
         r = run(ctx, poses, flags)
         rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         for k in ("e", "v"):
@@ -155,6 +162,46 @@ def test_tile_and_order_invariance(small):
     r2 = run(ctx2, poses, flags)
     assert np.array_equal(r2["e"], base["e"]) and np.array_equal(r2["v"], base["v"])
     ctx2.close()
+
+
+def _list_entries_host(xyz, h):
+    """Neighbour-list size by brute force in f32: per node of the list grid, the
+    receptor atoms within rad = sqrt(100.5) + h*sqrt(3)/2 + 0.01 of the node,
+    padded to a multiple of 4 (the library's list layout)."""
+    f = np.float32
+    rad = f(np.sqrt(f(100.5))) + f(0.5) * np.sqrt(f(3.0)) * f(h) + f(0.01)
+    lo, hi = xyz.min(0), xyz.max(0)
+    o = np.floor(lo - float(rad) - h).astype(np.float32)
+    n = (np.ceil((hi + float(rad) + h - o.astype(np.float64)) / h) + 1).astype(int)
+    r = xyz.astype(np.float32)
+    ax = [o[a] + np.arange(n[a], dtype=np.float32) * f(h) for a in range(3)]
+    total = 0
+    for z in ax[2]:
+        dz2 = (z - r[:, 2]) ** 2
+        for y in ax[1]:
+            dy2 = (y - r[:, 1]) ** 2
+            dx = ax[0][:, None] - r[None, :, 0]
+            d2 = (dx * dx + dy2[None, :]) + dz2[None, :]
+            c = (d2 <= rad * rad).sum(1)
+            total += int(((c + 3) // 4 * 4).sum())
+    return total, tuple(int(v) for v in n)
+
+
+def test_list_build_counts(gpu):
+    """The device-built neighbour lists hold exactly the f32 brute-force
+    membership (list grid dims and padded entry totals)."""
+    cx = synth.make_config("tiny")
+    ctx = lklight.Context(cx, with_field=False)
+    for h in (1.0, 2.0):
+        say("list build counts, spacing", h)
+        ctx.set_list_spacing(h)
+        got, dims = ctx.list_info()
+        want, wdims = _list_entries_host(cx.rec.xyz, h)
+        assert dims == wdims, (h, dims, wdims)
+        assert abs(got - want) <= 8, (h, got, want)  # a pair exactly on the f32 boundary may flip
+    ctx.set_list_spacing(0.0)
+    assert ctx.list_info()[0] == 0
+    ctx.close()
 
 
 def test_determinism_and_batch_split(small):
