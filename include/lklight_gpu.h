@@ -178,6 +178,17 @@ int lkl_ctx_set_work_sort(lkl_ctx* ctx, int on);
  * src/gpu_score.rs:61-112 there). */
 int lkl_ctx_set_list_spacing(lkl_ctx* ctx, float spacing);
 
+/* Far-field layout (default on): besides phi, keep a corner-packed copy (the
+ * 8 corners of every grid cell together, 32 bytes per cell, 8x the memory of
+ * phi) so a trilinear sample is one aligned read.  Samples are bitwise those
+ * of phi; when the copy cannot be allocated phi is sampled directly.
+ * Library-internal layout with no counterpart in the reference (grid_dna.rs
+ * keeps phi only). */
+int lkl_ctx_set_phi_packed(lkl_ctx* ctx, int on);
+
+/* 1 when the context holds the corner-packed far field, else 0. */
+int lkl_ctx_phi_packed(const lkl_ctx* ctx);
+
 /* Neighbour-list state: returns the number of list entries (0 = lists off,
  * -1 = null context) and the node grid dimensions in dims3 (may be NULL). */
 long long lkl_ctx_list_info(const lkl_ctx* ctx, int* dims3);

@@ -71,6 +71,11 @@ struct DevComplex {
     const unsigned short* nidx;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
+    // work-ordered tiles over lists: each atom's list is cut into segments of
+    // at most seg entries (a multiple of LIST_PAD; 0 = one item per atom), at
+    // most maxseg per atom (sizes the tile's item array)
+    unsigned seg;
+    int maxseg;
     // ligand, spatially sorted (Morton order of the reference coordinates)
     const float4* lxyzq;
     const float4* lpar;
@@ -82,6 +87,10 @@ struct DevComplex {
     const float* phi;
     int nx, ny, nz;
     double ox, oy, oz, sp;
+    // corner-packed copy (NULL = sample phi): per cell (ix, iy, iz) < (nx-1,
+    // ny-1, nz-1), x fastest, two float4 {c000, c100, c010, c110} and
+    // {c001, c101, c011, c111} -- the cell's 8 corners in one 32-byte read
+    const float4* phi8;
 };
 
 struct Mat {
@@ -140,13 +149,21 @@ __device__ __forceinline__ double sample_phi(const DevComplex& C, double x, doub
     const double tx = (double)(float)(fx - ix);
     const double ty = (double)(float)(fy - iy);
     const double tz = (double)(float)(fz - iz);
-    const size_t sy = (size_t)C.nx, sz = (size_t)C.nx * (size_t)C.ny;
-    const size_t b = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
-    const float* __restrict__ g = C.phi;
-    const double c000 = g[b], c100 = g[b + 1];
-    const double c010 = g[b + sy], c110 = g[b + sy + 1];
-    const double c001 = g[b + sz], c101 = g[b + sz + 1];
-    const double c011 = g[b + sz + sy], c111 = g[b + sz + sy + 1];
+    double c000, c100, c010, c110, c001, c101, c011, c111;
+    if (C.phi8) {
+        const size_t c = ((size_t)iz * (size_t)(C.ny - 1) + (size_t)iy) * (size_t)(C.nx - 1) + (size_t)ix;
+        const float4 lo = C.phi8[2 * c], hi = C.phi8[2 * c + 1];
+        c000 = lo.x; c100 = lo.y; c010 = lo.z; c110 = lo.w;
+        c001 = hi.x; c101 = hi.y; c011 = hi.z; c111 = hi.w;
+    } else {
+        const size_t sy = (size_t)C.nx, sz = (size_t)C.nx * (size_t)C.ny;
+        const size_t b = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+        const float* __restrict__ g = C.phi;
+        c000 = g[b]; c100 = g[b + 1];
+        c010 = g[b + sy]; c110 = g[b + sy + 1];
+        c001 = g[b + sz]; c101 = g[b + sz + 1];
+        c011 = g[b + sz + sy]; c111 = g[b + sz + sy + 1];
+    }
     const double c00 = c000 * (1.0 - tx) + c100 * tx;
     const double c10 = c010 * (1.0 - tx) + c110 * tx;
     const double c01 = c001 * (1.0 - tx) + c101 * tx;

@@ -453,6 +453,105 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
     __syncthreads();  // the LDS lists may be reused by the caller
 }
 
+// Work-ordered near field over neighbour lists with list segments as the work
+// items: pass 1 transforms every atom of [j0, j1), adds its far field, looks up
+// its list [b, e) and counts its segments (at most C.seg entries each) into 16
+// buckets by descending segment length; the items are then scattered in bucket
+// order and pass 2 walks one segment per item, so a long list is shared by
+// several lanes and the lanes of a wavefront walk segments of similar length.
+// With desolvation, each item's ligand-side minimum meets the atom's in LDS
+// (64-bit atomic min on the f64 bits, exact) and pass 3 adds the per-atom
+// terms.  All threads of the block must call it (barriers).  LDS (8-byte
+// aligned): [minL u64 x nT if DES][b u32 x nT][e u32 x nT][items u32 x
+// nT * C.maxseg].
+constexpr int SEG_BKT = 16;
+
+template <bool FAR, bool ELEC, bool CLASH, int DES>
+__device__ __forceinline__ void near_segments(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
+                                              unsigned* __restrict__ minR, long long& e, long long& v,
+                                              long long& s) {
+    __shared__ unsigned bcnt[SEG_BKT], boff[SEG_BKT];
+    const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+    const int nT = j1 - j0;
+    const unsigned seg = C.seg;
+    unsigned long long* minL = reinterpret_cast<unsigned long long*>(lds);
+    unsigned* ab = reinterpret_cast<unsigned*>(DES != DES_NONE ? (void*)(minL + nT) : lds);
+    unsigned* ae = ab + nT;
+    unsigned* items = ae + nT;
+    // bucket of a segment of n entries (1 <= n <= seg): 0 for full, larger for shorter
+    auto seg_bucket = [&](unsigned n) -> int {
+        return (int)(((seg - n) * (unsigned)SEG_BKT) / (seg + 1u));
+    };
+    for (int k = tid; k < SEG_BKT; k += bs) bcnt[k] = 0u;
+    __syncthreads();
+    for (int t = tid; t < nT; t += bs) {
+        const float4 lb = C.lxyzq[j0 + t];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
+        const int n = list_node(C, x, y, z);
+        unsigned b = 0u, en = 0u;
+        if (n >= 0) {
+            b = C.noff[n];
+            en = C.noff[n + 1];
+        }
+        ab[t] = b;
+        ae[t] = en;
+        if (DES != DES_NONE) minL[t] = (unsigned long long)__double_as_longlong(MIN_INIT);
+        const unsigned len = en - b;
+        if (len) {
+            const unsigned full = len / seg, rest = len % seg;
+            if (full) atomicAdd(&bcnt[0], full);
+            if (rest) atomicAdd(&bcnt[seg_bucket(rest)], 1u);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        unsigned acc0 = 0u;
+        for (int k = 0; k < SEG_BKT; ++k) {
+            boff[k] = acc0;
+            acc0 += bcnt[k];
+        }
+        bcnt[0] = acc0;  // total items (bcnt is not read again)
+    }
+    __syncthreads();
+    for (int t = tid; t < nT; t += bs) {
+        const unsigned len = ae[t] - ab[t];
+        const unsigned full = len / seg, rest = len % seg;
+        if (full) {
+            const unsigned o = atomicAdd(&boff[0], full);
+            for (unsigned q = 0; q < full; ++q) items[o + q] = (unsigned)t | (q << 16);
+        }
+        if (rest) items[atomicAdd(&boff[seg_bucket(rest)], 1u)] = (unsigned)t | (full << 16);
+    }
+    __syncthreads();
+    const int nitems = (int)bcnt[0];
+    for (int k = tid; k < nitems; k += bs) {
+        const unsigned it = items[k];
+        const int t = (int)(it & 0xffffu);
+        const unsigned q = it >> 16;
+        const int j = j0 + t;
+        const float4 lb = C.lxyzq[j];
+        const float4 lp = C.lpar[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        const unsigned b = ab[t] + q * seg;
+        const unsigned en = min(b + seg, ae[t]);
+        double mloc = MIN_INIT;
+        const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
+        near_list<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, b, en, e, v, mloc, minR);
+        if (DES != DES_NONE && mloc < MIN_INIT)
+            atomicMin(&minL[t], (unsigned long long)__double_as_longlong(mloc));
+    }
+    __syncthreads();
+    if (DES != DES_NONE) {
+        for (int t = tid; t < nT; t += bs)
+            s += to_fix(solv_term(__longlong_as_double((long long)minL[t]), C.lasa[j0 + t], C.ldes[j0 + t]),
+                        E_SCALE);
+        __syncthreads();
+    }
+}
+
 // One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
 // out[2k+1] = V, outS[k] = S.  With several chunks per pose the tiles add their
 // fixed-point partials into acc[3k..3k+2] (integer atomics: exact, order-free);
@@ -468,7 +567,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
              float* __restrict__ pa_v, long long* __restrict__ acc, unsigned* __restrict__ cnt,
              int chunk, int nchunks) {
-    extern __shared__ unsigned lds_min[];
+    extern __shared__ __attribute__((aligned(16))) unsigned lds_min[];
     __shared__ long long red[3][16];
     const unsigned total = (unsigned)N * (unsigned)nchunks;
     if (blockIdx.x >= total) return;
@@ -492,8 +591,12 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     long long e = 0, v = 0, s = 0;
     if (!PERATOM && C.wgrid) {
         // lists after the receptor minima when those live in LDS
-        void* lists = DES == DES_LDS ? (void*)(lds_min + C.nr) : (void*)lds_min;
-        near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+        void* lists = DES == DES_LDS ? (void*)(((uintptr_t)(lds_min + C.nr) + 7u) & ~(uintptr_t)7u)
+                                     : (void*)lds_min;
+        if (LIST && C.seg)
+            near_segments<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+        else
+            near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
     } else if constexpr (DES == DES_NONE && !PERATOM) {
         {
             for (int j = j0 + tid; j < j1; j += bs) {
@@ -580,7 +683,10 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
                                           long long& e, long long& v) {
     if (C.wgrid) {
         long long s_unused = 0;
-        near_sorted<LIST, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+        if (LIST && C.seg)
+            near_segments<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+        else
+            near_sorted<LIST, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
     }
     for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
@@ -705,6 +811,7 @@ struct lkl_ctx {
     int nn[3] = {0, 0, 0};
     float no[3] = {0.f, 0.f, 0.f};
     size_t list_entries = 0;  // 0 = no lists (row walk)
+    unsigned list_max = 0;    // longest (padded) list
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -712,6 +819,8 @@ struct lkl_ctx {
     bool has_phi = false, has_desolv = false;
     // device
     float* d_phi = nullptr;
+    float4* d_phi8 = nullptr;  // corner-packed copy of phi (NULL = off / not built)
+    bool phi_packed = true;
     int nx = 0, ny = 0, nz = 0;
     double ox = 0, oy = 0, oz = 0, sp = 1;
     float4 *d_rxyzq = nullptr, *d_rpar = nullptr;
@@ -769,6 +878,40 @@ static int dev_reserve(T** buf, size_t* cap, size_t n) {
 // atoms stably counting-sorted by cell (x fastest), packed float4 records.
 static float cell_x_of(const lkl_ctx* c) { return c->cell_x > 0.0f ? c->cell_x : c->cell_size; }
 
+// Corner-packed far field: for every cell of phi, its 8 corner values in the
+// order sample_phi reads them, so one 32-byte read serves a sample.  Values
+// are copied, not recomputed: samples are bitwise those of phi.
+__global__ void phi_pack_kernel(const float* __restrict__ phi, int nx, int ny, int nz, float4* __restrict__ out) {
+    const long long mx = nx - 1, my = ny - 1, mz = nz - 1;
+    const long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (c >= mx * my * mz) return;
+    const long long ix = c % mx, t = c / mx, iy = t % my, iz = t / my;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * (size_t)ny;
+    const size_t b = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+    out[2 * c] = make_float4(phi[b], phi[b + 1], phi[b + sy], phi[b + sy + 1]);
+    out[2 * c + 1] = make_float4(phi[b + sz], phi[b + sz + 1], phi[b + sz + sy], phi[b + sz + sy + 1]);
+}
+
+// (Re)build the packed copy of the context's phi when packing is on; an
+// allocation failure leaves it off (sample_phi then reads phi itself).
+static int build_phi8(lkl_ctx* c) {
+    if (c->d_phi8) { (void)hipFree(c->d_phi8); c->d_phi8 = nullptr; }
+    if (!c->phi_packed || !c->has_phi || !c->d_phi) return 0;
+    const long long cells = (long long)(c->nx - 1) * (c->ny - 1) * (c->nz - 1);
+    if (cells <= 0) return 0;
+    if (hipMalloc((void**)&c->d_phi8, sizeof(float4) * 2 * (size_t)cells) != hipSuccess) {
+        (void)hipGetLastError();
+        c->d_phi8 = nullptr;
+        return 0;
+    }
+    hipLaunchKernelGGL(phi_pack_kernel, dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, c->stream, c->d_phi,
+                       c->nx, c->ny, c->nz, c->d_phi8);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    c->version++;
+    return 0;
+}
+
 // Neighbour lists over the sorted receptor (indices into d_rxyzq).  Lists are
 // skipped (row walk) when switched off, when an index would not fit 16 bits, or
 // when the node grid or the entry count exceeds the caps below.
@@ -779,6 +922,7 @@ static int build_lists(lkl_ctx* c) {
     if (c->d_noff) { (void)hipFree(c->d_noff); c->d_noff = nullptr; }
     if (c->d_nidx) { (void)hipFree(c->d_nidx); c->d_nidx = nullptr; }
     c->list_entries = 0;
+    c->list_max = 0;
     const int nr = c->nr;
     if (!(c->list_h > 0.0f) || nr <= 0 || nr > 65535) return 0;
     const float h = c->list_h;
@@ -807,8 +951,10 @@ static int build_lists(lkl_ctx* c) {
     std::vector<unsigned> off((size_t)nodes + 1);
     HIPCK("readback", hipMemcpy(off.data(), c->d_noff, sizeof(unsigned) * (size_t)nodes, hipMemcpyDeviceToHost));
     unsigned long long acc = 0;
+    unsigned lmax = 0;
     for (long long k = 0; k < nodes; ++k) {
         const unsigned cnt = off[k];
+        lmax = std::max(lmax, cnt);
         off[k] = (unsigned)acc;
         acc += cnt;
         if (acc > (unsigned long long)LIST_ENTRIES_MAX) break;
@@ -827,6 +973,7 @@ static int build_lists(lkl_ctx* c) {
     HIPCK("sync", hipStreamSynchronize(c->stream));
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
     c->list_entries = (size_t)std::max(acc, 1ull);
+    c->list_max = lmax;
     c->version++;
     return 0;
 }
@@ -1029,6 +1176,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.lasa = c->d_lasa;
     C.nl = c->nl;
     C.phi = c->d_phi;
+    C.phi8 = c->d_phi8;
     C.nx = c->nx;
     C.ny = c->ny;
     C.nz = c->nz;
@@ -1042,7 +1190,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
 static void ctx_free(lkl_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* bufs[] = {c->d_phi, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
+    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx};
@@ -1099,6 +1247,7 @@ static lkl_ctx* ctx_create(const lkl_complex_desc* d) {
         c->nx = d->nx; c->ny = d->ny; c->nz = d->nz;
         c->ox = d->ox; c->oy = d->oy; c->oz = d->oz; c->sp = d->sp;
         if (dev_upload(&c->d_phi, d->phi, (size_t)d->nx * d->ny * d->nz)) return fail();
+        if (build_phi8(c)) return fail();
     }
     if (build_receptor(c)) return fail();
     if (build_ligand(c, d->l_xyz, d->l_ele, d->l_svdw, d->l_vdwr, d->l_heavy,
@@ -1146,6 +1295,27 @@ static int validate_flags(const lkl_ctx* c, unsigned flags) {
     return 0;
 }
 
+// Segment plan of a work-ordered list tile of nT ligand atoms: the shortest
+// segment length (from 64 entries, doubling) whose item array fits `budget`
+// bytes of LDS together with the per-atom arrays (near_segments' layout).
+// Returns the bytes needed, or 0 when no length fits (one item per atom then).
+constexpr unsigned SEG_MIN = 64;
+constexpr size_t TILE_LDS_BUDGET = 53 * 1024;  // three 512-thread tiles per CU
+
+static size_t plan_segments(size_t nT, unsigned list_max, bool des, size_t budget, unsigned* seg, int* maxseg) {
+    const unsigned lm = std::max(list_max, (unsigned)LIST_PAD);
+    for (unsigned sg = SEG_MIN;; sg *= 2) {
+        const size_t ms = (lm + sg - 1) / sg;
+        const size_t need = (des ? 8 * nT : 0) + 8 * nT + 4 * nT * ms + 16;
+        if (need <= budget) {
+            *seg = sg;
+            *maxseg = (int)ms;
+            return need;
+        }
+        if (sg >= lm) return 0;
+    }
+}
+
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
                         double* d_outS, hipStream_t st, bool peratom) {
@@ -1187,6 +1357,15 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         shm = (size_t)chunk * 3 + 16;                      // index list + bucket ids
     }
     if (peratom) C.wgrid = nullptr;
+    if (C.wgrid && C.noff) {  // list segments as work items when their arrays fit
+        const size_t base = des == DES_LDS ? (size_t)c->nr * sizeof(unsigned) + 8 : 0;
+        const size_t budget = std::min(c->lds_max - std::min(c->lds_max, (size_t)1024), TILE_LDS_BUDGET);
+        if (base < budget) {
+            const size_t need = plan_segments((size_t)chunk, c->list_max, des != DES_NONE, budget - base, &C.seg,
+                                              &C.maxseg);
+            if (need) shm = base + need;
+        }
+    }
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
@@ -1263,6 +1442,7 @@ LKL_API int lkl_ctx_build_field(lkl_ctx* c) {
     const long long total = lkl_field_dims(c->r_xyz.data(), c->nr, o, d);
     if (total <= 0) { set_err("lkl_ctx_build_field: bad field box"); return -1; }
     if (c->d_phi) { (void)hipFree(c->d_phi); c->d_phi = nullptr; }
+    if (c->d_phi8) { (void)hipFree(c->d_phi8); c->d_phi8 = nullptr; }
     c->has_phi = false;
     HIPCK("alloc", hipMalloc((void**)&c->d_phi, sizeof(float) * (size_t)total));
     if (lkl_field_build_device(c->r_xyz.data(), c->r_ele.data(), c->nr, c->d_phi, c->stream)) {
@@ -1273,7 +1453,7 @@ LKL_API int lkl_ctx_build_field(lkl_ctx* c) {
     c->ox = o[0]; c->oy = o[1]; c->oz = o[2]; c->sp = 0.5;
     c->has_phi = true;
     c->version++;
-    return 0;
+    return build_phi8(c);
 }
 
 LKL_API int lkl_ctx_field_info(const lkl_ctx* c, double* origin3, int* dims3) {
@@ -1328,6 +1508,19 @@ LKL_API int lkl_ctx_set_work_sort(lkl_ctx* c, int on) {
     }
     return 0;
 }
+
+LKL_API int lkl_ctx_set_phi_packed(lkl_ctx* c, int on) {
+    if (!c) { set_err("lkl_ctx_set_phi_packed: null context"); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    if (c->phi_packed != (on != 0)) {
+        c->phi_packed = on != 0;
+        if (build_phi8(c)) return -1;
+        c->version++;
+    }
+    return 0;
+}
+
+LKL_API int lkl_ctx_phi_packed(const lkl_ctx* c) { return c && c->d_phi8 ? 1 : 0; }
 
 LKL_API int lkl_ctx_set_list_spacing(lkl_ctx* c, float spacing) {
     if (!c) { set_err("lkl_ctx_set_list_spacing: null context"); return -1; }
@@ -1506,7 +1699,10 @@ static int batch_refresh(lkl_multi* b) {
         int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
         if (cx[r].wgrid) {
             chunk = std::min(chunk, WORK_TILE_MAX);  // LDS list bound
-            b->lds = std::max(b->lds, (size_t)chunk * 3 + 16);
+            size_t need = 0;
+            if (cx[r].noff)
+                need = plan_segments((size_t)chunk, c->list_max, false, TILE_LDS_BUDGET, &cx[r].seg, &cx[r].maxseg);
+            b->lds = std::max(b->lds, need ? need : (size_t)chunk * 3 + 16);
         }
         pd[r].tile_begin = tile;
         pd[r].nchunks = (c->nl + chunk - 1) / chunk;

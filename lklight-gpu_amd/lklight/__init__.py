@@ -149,6 +149,13 @@ class Context:
         """lkl_ctx_set_list_spacing: neighbour-list node spacing (A); 0 = cell-row walk."""
         _capi.check(lib().lkl_ctx_set_list_spacing(self.h, float(spacing)), "lkl_ctx_set_list_spacing")
 
+    def set_phi_packed(self, on):
+        """lkl_ctx_set_phi_packed: keep the corner-packed far field (8 corners per cell)."""
+        _capi.check(lib().lkl_ctx_set_phi_packed(self.h, int(bool(on))), "lkl_ctx_set_phi_packed")
+
+    def phi_packed(self):
+        return bool(lib().lkl_ctx_phi_packed(self.h))
+
     def list_info(self):
         """(list entries, node grid dims); entries 0 = lists off."""
         d = np.zeros(3, np.int32)

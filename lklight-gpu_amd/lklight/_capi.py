@@ -80,6 +80,8 @@ def lib():
         "lkl_ctx_set_cell_x": (C.c_int, [C.c_void_p, C.c_float]),
         "lkl_ctx_set_work_sort": (C.c_int, [C.c_void_p, C.c_int]),
         "lkl_ctx_set_list_spacing": (C.c_int, [C.c_void_p, C.c_float]),
+        "lkl_ctx_set_phi_packed": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_ctx_phi_packed": (C.c_int, [C.c_void_p]),
         "lkl_ctx_list_info": (C.c_longlong, [C.c_void_p, C.c_void_p]),
         "lkl_batch_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                       C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int,

@@ -122,18 +122,21 @@ def test_tile_and_order_invariance(small):
     say("tile invariance")
     ctx = lklight.Context(cx, desolv=True)
     flags = lklight.FLAGS_DNA
-    ctx.set_list_spacing(0.0)  # reference run: cell-row walk
-    assert ctx.list_info()[0] == 0
+    ctx.set_list_spacing(0.0)  # reference run: cell-row walk, phi sampled directly
+    ctx.set_phi_packed(False)
+    assert ctx.list_info()[0] == 0 and not ctx.phi_packed()
     base = run(ctx, poses, flags)
     base_c = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
-    for cell, block, chunk, cell_x, lh in [(10.0, 256, 0, 0.0, 1.0), (3.4, 64, 64, 0.0, 0.0),
-                                           (6.0, 512, 0, 0.0, 0.5), (4.0, 128, 256, 0.0, 2.0),
-                                           (5.0, 64, 128, 0.0, 1.5), (5.0, 256, 0, 1.25, 0.0),
-                                           (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0)]:
-        say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh}")
+    for i, (cell, block, chunk, cell_x, lh) in enumerate(
+            [(10.0, 256, 0, 0.0, 1.0), (3.4, 64, 64, 0.0, 0.0), (6.0, 512, 0, 0.0, 0.5),
+             (4.0, 128, 256, 0.0, 2.0), (5.0, 64, 128, 0.0, 1.5), (5.0, 256, 0, 1.25, 0.0),
+             (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0)]):
+        packed = i % 2 == 0
+        say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh} packed phi {packed}")
         ctx.set_tiles(cell, block, chunk, cell_x=cell_x)
         ctx.set_list_spacing(lh)
-        assert (ctx.list_info()[0] > 0) == (lh > 0)
+        ctx.set_phi_packed(packed)
+        assert (ctx.list_info()[0] > 0) == (lh > 0) and ctx.phi_packed() == packed
         r = run(ctx, poses, flags)
         rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         for k in ("e", "v"):
@@ -361,6 +364,14 @@ def test_config2_full_size(gpu):
     # batch split and repeat invariance at full size
     again = run(ctx, poses[600:], lklight.FLAGS_DNA)
     assert np.array_equal(again["e"], full["e"][600:]) and np.array_equal(again["v"], full["v"][600:])
+    # the default layouts (neighbour lists, corner-packed phi) against the
+    # cell-row walk sampling phi directly: the same bits at full size
+    assert ctx.list_info()[0] > 0 and ctx.phi_packed()
+    ctx.set_list_spacing(0.0)
+    ctx.set_phi_packed(False)
+    say("c2 score 1200 poses, cell-row walk and unpacked phi")
+    rows = run(ctx, poses, lklight.FLAGS_DNA)
+    assert np.array_equal(rows["e"], full["e"]) and np.array_equal(rows["v"], full["v"])
     ctx.close()
 
 
