@@ -69,6 +69,9 @@ struct DevComplex {
     // far from everything).  noff holds nodes + 1 offsets into nidx.
     const unsigned* noff;
     const unsigned short* nidx;
+    // the sorted receptor records interleaved for the list walk: rrec[2p] =
+    // rxyzq[p], rrec[2p + 1] = rpar[p] (one 32-byte record per atom, nr + 1)
+    const float4* rrec;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
     // work-ordered tiles over lists: each atom's list is cut into segments of

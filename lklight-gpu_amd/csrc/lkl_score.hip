@@ -90,12 +90,14 @@ constexpr int NU = LKL_NU;
 // z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
 // directly in fixed-point units, and (with desolvation) the per-atom minima of
 // cpydock.rs:620-627.  (xh + xl) is the hi/lo f32 split of the f64 coordinate.
-template <bool ELEC, bool CLASH, int DES>
+// With PRE the caller has already loaded the candidate's parameters (rp_pre);
+// otherwise they are loaded for the pairs within the cutoff only.
+template <bool ELEC, bool CLASH, int DES, bool PRE = false>
 __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, int p, double x64, double y64,
                                           double z64, float xh, float yh, float zh, float xl, float yl,
                                           float zl, float qjs, float svjs, float rj, bool hj, bool lf,
                                           long long& e, long long& v, double& minL,
-                                          unsigned* __restrict__ minR) {
+                                          unsigned* __restrict__ minR, float4 rp_pre = float4()) {
     const float dx = (xh - r.x) + xl;
     const float dy = (yh - r.y) + yl;
     const float dz = (zh - r.z) + zl;
@@ -108,7 +110,7 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
         }
     }
     if (d2 <= NEAR_HI && !(d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2)) {
-        const float4 rp = C.rpar[p];
+        const float4 rp = PRE ? rp_pre : C.rpar[p];
         const float inv = __builtin_amdgcn_rcpf(d2);
         if (ELEC) {  // clamp to +-ES_CAP (selects, no branches)
             const float ae = qjs * r.w * inv;
@@ -204,28 +206,47 @@ __device__ __forceinline__ int list_node(const DevComplex& C, double x, double y
 // aligned 8-byte load gives LIST_PAD receptor indices, whose records are then
 // requested together; padding entries point at the dummy record, which fails
 // every distance test.
+// Near field of one ligand atom by its node's neighbour list [b, e): each step
+// takes two aligned 8-byte loads of LIST_PAD receptor indices (the second
+// replaced by padding past e) and then requests all 8 interleaved records
+// (coordinates and parameters, 32 bytes each) together, so a wavefront keeps
+// 8 gathers per lane in flight; padding entries point at the dummy record,
+// which fails every distance test.
 template <bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_list(const DevComplex& C, double x64, double y64, double z64,
                                           float qj, float svj, float rj, bool hj, bool lf, unsigned b,
                                           unsigned en, long long& e, long long& v, double& minL,
                                           unsigned* __restrict__ minR) {
-    static_assert(LIST_PAD == 4, "one ushort4 per step");
+    static_assert(LIST_PAD == 4, "one ushort4 per list vector");
+    constexpr int U = 2;  // list vectors per step
     const float qjs = qj * E_SCALE_F;
     const float svjs = svj * V_SCALE_F;
     const float xh = (float)x64, yh = (float)y64, zh = (float)z64;
     const float xl = (float)(x64 - (double)xh);
     const float yl = (float)(y64 - (double)yh);
     const float zl = (float)(z64 - (double)zh);
-    for (unsigned p0 = b; p0 < en; p0 += LIST_PAD) {
-        const ushort4 q = *reinterpret_cast<const ushort4*>(C.nidx + p0);
-        const int id[4] = {(int)q.x, (int)q.y, (int)q.z, (int)q.w};
-        float4 rq[4];
+    const unsigned short dn = (unsigned short)C.nr;  // the dummy record
+    for (unsigned p0 = b; p0 < en; p0 += 4u * U) {
+        int id[4 * U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) rq[k] = C.rxyzq[id[k]];
+        for (int u = 0; u < U; ++u) {
+            ushort4 q = make_ushort4(dn, dn, dn, dn);
+            if (u == 0 || p0 + 4u * u < en) q = *reinterpret_cast<const ushort4*>(C.nidx + p0 + 4u * u);
+            id[4 * u] = q.x;
+            id[4 * u + 1] = q.y;
+            id[4 * u + 2] = q.z;
+            id[4 * u + 3] = q.w;
+        }
+        float4 rq[4 * U], rp[4 * U];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            pair_term<ELEC, CLASH, DES>(C, rq[k], id[k], x64, y64, z64, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
-                                        hj, lf, e, v, minL, minR);
+        for (int k = 0; k < 4 * U; ++k) {
+            rq[k] = C.rrec[2 * id[k]];
+            rp[k] = C.rrec[2 * id[k] + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4 * U; ++k)
+            pair_term<ELEC, CLASH, DES, true>(C, rq[k], id[k], x64, y64, z64, xh, yh, zh, xl, yl, zl, qjs, svjs,
+                                              rj, hj, lf, e, v, minL, minR, rp[k]);
     }
 }
 
@@ -823,7 +844,7 @@ struct lkl_ctx {
     bool phi_packed = true;
     int nx = 0, ny = 0, nz = 0;
     double ox = 0, oy = 0, oz = 0, sp = 1;
-    float4 *d_rxyzq = nullptr, *d_rpar = nullptr;
+    float4 *d_rxyzq = nullptr, *d_rpar = nullptr, *d_rrec = nullptr;
     int* d_cstart = nullptr;
     unsigned char* d_rflag = nullptr;
     float *d_rdes = nullptr, *d_rasa = nullptr;
@@ -856,12 +877,20 @@ int lkl_field_build_device(const float* r_xyz, const float* r_ele, int nr, float
                            hipStream_t st);
 extern "C" long long lkl_field_dims(const float* r_xyz, int nr, double* origin3, int* dims3);
 
+// Host-to-device copy ordered on `st` and waited for.  A plain hipMemcpy from
+// pageable memory may return before the data lands, and the kernels run on
+// non-blocking streams that do not wait for the null stream.
+static hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+
 template <typename T>
-static int dev_upload(T** dst, const T* src, size_t n) {
+static int dev_upload(T** dst, const T* src, size_t n, hipStream_t st) {
     if (*dst) { (void)hipFree(*dst); *dst = nullptr; }
     if (n == 0) return 0;
     HIPCK("alloc", hipMalloc((void**)dst, n * sizeof(T)));
-    HIPCK("upload", hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(*dst, src, n * sizeof(T), st));
     return 0;
 }
 
@@ -965,7 +994,7 @@ static int build_lists(lkl_ctx* c) {
         return 0;
     }
     off[nodes] = (unsigned)acc;
-    HIPCK("upload", hipMemcpy(c->d_noff, off.data(), sizeof(unsigned) * off.size(), hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(c->d_noff, off.data(), sizeof(unsigned) * off.size(), c->stream));
     HIPCK("alloc", hipMalloc((void**)&c->d_nidx, sizeof(unsigned short) * (size_t)std::max(acc, 4ull)));
     hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, nr, o[0], o[1], o[2], h,
                        n[0], n[1], n[2], rad, c->d_noff, c->d_nidx);
@@ -1031,13 +1060,21 @@ static int build_receptor(lkl_ctx* c) {
             asa[p] = c->r_asa[i];
         }
     }
-    if (dev_upload(&c->d_rxyzq, xyzq.data(), (size_t)nr + 1)) return -1;
-    if (dev_upload(&c->d_rpar, par.data(), (size_t)nr + 1)) return -1;
-    if (dev_upload(&c->d_cstart, start.data(), ncell + 1)) return -1;
+    if (dev_upload(&c->d_rxyzq, xyzq.data(), (size_t)nr + 1, c->stream)) return -1;
+    if (dev_upload(&c->d_rpar, par.data(), (size_t)nr + 1, c->stream)) return -1;
+    {
+        std::vector<float4> rec(2 * ((size_t)nr + 1));
+        for (size_t p = 0; p <= (size_t)nr; ++p) {
+            rec[2 * p] = xyzq[p];
+            rec[2 * p + 1] = par[p];
+        }
+        if (dev_upload(&c->d_rrec, rec.data(), rec.size(), c->stream)) return -1;
+    }
+    if (dev_upload(&c->d_cstart, start.data(), ncell + 1, c->stream)) return -1;
     if (c->has_desolv) {
-        if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr + 1)) return -1;
-        if (dev_upload(&c->d_rdes, des.data(), (size_t)nr)) return -1;
-        if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr)) return -1;
+        if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr + 1, c->stream)) return -1;
+        if (dev_upload(&c->d_rdes, des.data(), (size_t)nr, c->stream)) return -1;
+        if (dev_upload(&c->d_rasa, asa.data(), (size_t)nr, c->stream)) return -1;
     }
     if (!c->d_wgrid && nr > 0) {  // receptor-only: built once
         double wlo[3], whi[3];
@@ -1118,12 +1155,12 @@ static int build_ligand(lkl_ctx* c, const float* xyz, const float* ele, const fl
             la[s] = asa[j];
         }
     }
-    if (dev_upload(&c->d_lxyzq, lx.data(), (size_t)nl)) return -1;
-    if (dev_upload(&c->d_lpar, lp.data(), (size_t)nl)) return -1;
+    if (dev_upload(&c->d_lxyzq, lx.data(), (size_t)nl, c->stream)) return -1;
+    if (dev_upload(&c->d_lpar, lp.data(), (size_t)nl, c->stream)) return -1;
     if (flag) {
-        if (dev_upload(&c->d_lflag, lf.data(), (size_t)nl)) return -1;
-        if (dev_upload(&c->d_ldes, ld.data(), (size_t)nl)) return -1;
-        if (dev_upload(&c->d_lasa, la.data(), (size_t)nl)) return -1;
+        if (dev_upload(&c->d_lflag, lf.data(), (size_t)nl, c->stream)) return -1;
+        if (dev_upload(&c->d_ldes, ld.data(), (size_t)nl, c->stream)) return -1;
+        if (dev_upload(&c->d_lasa, la.data(), (size_t)nl, c->stream)) return -1;
     }
     c->version++;
     return 0;
@@ -1134,6 +1171,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
     memset(&C, 0, sizeof C);
     C.rxyzq = c->d_rxyzq;
     C.rpar = c->d_rpar;
+    C.rrec = c->d_rrec;
     C.cstart = c->d_cstart;
     C.rflag = c->d_rflag;
     C.rdes = c->d_rdes;
@@ -1190,7 +1228,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
 static void ctx_free(lkl_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_cstart, c->d_rflag, c->d_rdes,
+    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx};
@@ -1246,7 +1284,7 @@ static lkl_ctx* ctx_create(const lkl_complex_desc* d) {
     if (c->has_phi) {
         c->nx = d->nx; c->ny = d->ny; c->nz = d->nz;
         c->ox = d->ox; c->oy = d->oy; c->oz = d->oz; c->sp = d->sp;
-        if (dev_upload(&c->d_phi, d->phi, (size_t)d->nx * d->ny * d->nz)) return fail();
+        if (dev_upload(&c->d_phi, d->phi, (size_t)d->nx * d->ny * d->nz, c->stream)) return fail();
         if (build_phi8(c)) return fail();
     }
     if (build_receptor(c)) return fail();
@@ -1268,8 +1306,11 @@ static int reserve_acc(lkl_ctx* c, size_t N) {
     c->cap_acc = 0;
     HIPCK("alloc", hipMalloc((void**)&c->d_acc, sizeof(long long) * 3 * N));
     HIPCK("alloc", hipMalloc((void**)&c->d_cnt, sizeof(unsigned) * N));
-    HIPCK("alloc", hipMemset(c->d_acc, 0, sizeof(long long) * 3 * N));
-    HIPCK("alloc", hipMemset(c->d_cnt, 0, sizeof(unsigned) * N));
+    // cleared on the context's stream and waited for: the first launch may be
+    // enqueued on another stream, which must not overtake the clear
+    HIPCK("alloc", hipMemsetAsync(c->d_acc, 0, sizeof(long long) * 3 * N, c->stream));
+    HIPCK("alloc", hipMemsetAsync(c->d_cnt, 0, sizeof(unsigned) * N, c->stream));
+    HIPCK("sync", hipStreamSynchronize(c->stream));
     c->cap_acc = N;
     return 0;
 }
@@ -1468,7 +1509,7 @@ LKL_API int lkl_ctx_transform(lkl_ctx* c, const double* poses, int N, double* ou
     HIPCK("probe", hipSetDevice(c->device));
     if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
     if (dev_reserve(&c->d_xyz, &c->cap_xyz, (size_t)N * c->nl * 3)) return -1;
-    HIPCK("upload", hipMemcpy(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, c->stream));
     hipLaunchKernelGGL(transform_kernel, dim3((unsigned)N), dim3(256), 0, c->stream, dev_view(c),
                        c->d_poses, N, c->d_xyz);
     HIPCK("launch", hipGetLastError());
@@ -1575,7 +1616,7 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
     if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
     if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
     if (dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
-    HIPCK("upload", hipMemcpy(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, c->stream));
     hipEvent_t e0, e1;
     HIPCK("alloc", hipEventCreate(&e0));
     HIPCK("alloc", hipEventCreate(&e1));
@@ -1713,8 +1754,8 @@ static int batch_refresh(lkl_multi* b) {
         b->versions[k] = c->version;
     }
     b->total_tiles = tile;
-    HIPCK("upload", hipMemcpy(b->d_cx, cx.data(), sizeof(DevComplex) * b->K, hipMemcpyHostToDevice));
-    HIPCK("upload", hipMemcpy(b->d_prob, pd.data(), sizeof(ProblemDesc) * b->K, hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(b->d_cx, cx.data(), sizeof(DevComplex) * b->K, b->stream));
+    HIPCK("upload", h2d(b->d_prob, pd.data(), sizeof(ProblemDesc) * b->K, b->stream));
     batch_drop_graph(b);
     return 0;
 }
@@ -1779,8 +1820,10 @@ LKL_API lkl_multi* lkl_multi_create(lkl_ctx* const* ctxs, const unsigned* flags,
         set_err("lkl_multi_create: allocation failed (%zu poses)", P);
         return fail();
     }
-    if (hipMemset(b->d_acc, 0, sizeof(long long) * 3 * P) != hipSuccess ||
-        hipMemset(b->d_cnt, 0, sizeof(unsigned) * P) != hipSuccess) {
+    // cleared on the batch's stream and waited for before any launch
+    if (hipMemsetAsync(b->d_acc, 0, sizeof(long long) * 3 * P, b->stream) != hipSuccess ||
+        hipMemsetAsync(b->d_cnt, 0, sizeof(unsigned) * P, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess) {
         set_err("lkl_multi_create: clearing counters failed");
         return fail();
     }
@@ -1868,7 +1911,7 @@ LKL_API int lkl_multi_autotune(lkl_multi* b, const double* poses, int* best_bloc
                                float* table, int* n_table) {
     if (!b || !poses) { set_err("lkl_multi_autotune: bad arguments"); return -1; }
     HIPCK("probe", hipSetDevice(b->device));
-    HIPCK("upload", hipMemcpy(b->d_poses, poses, sizeof(double) * 7 * (size_t)b->P, hipMemcpyHostToDevice));
+    HIPCK("upload", h2d(b->d_poses, poses, sizeof(double) * 7 * (size_t)b->P, b->stream));
     hipEvent_t e0, e1;
     HIPCK("alloc", hipEventCreate(&e0));
     HIPCK("alloc", hipEventCreate(&e1));

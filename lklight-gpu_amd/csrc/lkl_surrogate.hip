@@ -30,6 +30,14 @@ using namespace lkl;
 
 void lkl_set_error_text(const char* msg);  // lkl_score.hip
 
+// Host-to-device copy ordered on `st` and waited for.  A plain hipMemcpy from
+// pageable memory may return before the data lands, and the kernels run on
+// non-blocking streams that do not wait for the null stream.
+static hipError_t h2d(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e != hipSuccess ? e : hipStreamSynchronize(st);
+}
+
 namespace {
 
 constexpr int NG = 16, NB = 24, NF = NG * NG * NB;  // 6144 features
@@ -686,11 +694,11 @@ LKL_API lkl_feat* lkl_feat_create_grid(const float* r_xyz, const unsigned char* 
               hipMalloc((void**)&f->d_grid, gbytes) == hipSuccess &&
               hipMalloc((void**)&d_gs, sizeof(int) * (NG + 1)) == hipSuccess &&
               hipMalloc((void**)&d_over, 4) == hipSuccess &&
-              hipMemcpy(f->d_rec, r.data(), sizeof(float4) * nr, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(f->d_lig, l.data(), sizeof(float4) * nl, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(f->d_lut, lut.data(), 2 * LUT_N, hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d_gs, gstart.data(), sizeof(int) * (NG + 1), hipMemcpyHostToDevice) == hipSuccess &&
-              hipMemcpy(d_over, &over, 4, hipMemcpyHostToDevice) == hipSuccess;
+              h2d(f->d_rec, r.data(), sizeof(float4) * nr, f->stream) == hipSuccess &&
+              h2d(f->d_lig, l.data(), sizeof(float4) * nl, f->stream) == hipSuccess &&
+              h2d(f->d_lut, lut.data(), 2 * LUT_N, f->stream) == hipSuccess &&
+              h2d(d_gs, gstart.data(), sizeof(int) * (NG + 1), f->stream) == hipSuccess &&
+              h2d(d_over, &over, 4, f->stream) == hipSuccess;
     if (ok) {
         const long long threads = f->nodes * 16;
         hipLaunchKernelGGL(grid_build_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, f->stream,
@@ -762,9 +770,9 @@ LKL_API lkl_feat* lkl_feat_create(const float* r_xyz, const unsigned char* r_gro
         hipMalloc((void**)&f->d_rec, sizeof(float4) * nr) != hipSuccess ||
         hipMalloc((void**)&f->d_lig, sizeof(float4) * nl) != hipSuccess ||
         hipMalloc((void**)&f->d_lut, 2 * LUT_N) != hipSuccess ||
-        hipMemcpy(f->d_rec, r.data(), sizeof(float4) * nr, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(f->d_lig, l.data(), sizeof(float4) * nl, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(f->d_lut, lut.data(), 2 * LUT_N, hipMemcpyHostToDevice) != hipSuccess) {
+        h2d(f->d_rec, r.data(), sizeof(float4) * nr, f->stream) != hipSuccess ||
+        h2d(f->d_lig, l.data(), sizeof(float4) * nl, f->stream) != hipSuccess ||
+        h2d(f->d_lut, lut.data(), 2 * LUT_N, f->stream) != hipSuccess) {
         serr("lkl_feat_create: device setup failed");
         lkl_feat_destroy(f);
         return nullptr;
@@ -813,7 +821,7 @@ LKL_API int lkl_feat_compute_host(lkl_feat* f, const double* poses, int N, unsig
     if (hipMalloc((void**)&dp, sizeof(double) * 7 * N) != hipSuccess) rc = -1;
     if (!rc && out && hipMalloc((void**)&dout, 2 * (size_t)N * NF) != hipSuccess) rc = -1;
     if (!rc && counts && hipMalloc((void**)&dc, 4 * (size_t)N * NF) != hipSuccess) rc = -1;
-    if (!rc && hipMemcpy(dp, poses, sizeof(double) * 7 * N, hipMemcpyHostToDevice) != hipSuccess) rc = -1;
+    if (!rc && h2d(dp, poses, sizeof(double) * 7 * N, f->stream) != hipSuccess) rc = -1;
     if (!rc) rc = lkl_feat_compute(f, dp, N, dout, dc, f->stream);
     if (!rc && hipStreamSynchronize(f->stream) != hipSuccess) rc = -1;
     if (!rc && out && hipMemcpy(out, dout, 2 * (size_t)N * NF, hipMemcpyDeviceToHost) != hipSuccess) rc = -1;
@@ -919,8 +927,9 @@ LKL_API lkl_mlp* lkl_mlp_create(unsigned long long seed) {
         hipMalloc((void**)&n->m, NPARAM * 4) || hipMalloc((void**)&n->v, NPARAM * 4) ||
         hipMalloc((void**)&n->p16, NPARAM * 2) || hipMalloc((void**)&n->p16t, NT_W * 2) ||
         hipMalloc((void**)&n->loss, 4) ||
-        hipMemcpy(n->p, p.data(), NPARAM * 4, hipMemcpyHostToDevice) ||
-        hipMemset(n->m, 0, NPARAM * 4) || hipMemset(n->v, 0, NPARAM * 4) || hipMemset(n->g, 0, NPARAM * 4)) {
+        h2d(n->p, p.data(), NPARAM * 4, n->stream) ||
+        hipMemsetAsync(n->m, 0, NPARAM * 4, n->stream) || hipMemsetAsync(n->v, 0, NPARAM * 4, n->stream) ||
+        hipMemsetAsync(n->g, 0, NPARAM * 4, n->stream)) {
         serr("lkl_mlp_create: device setup failed");
         lkl_mlp_destroy(n);
         return nullptr;
@@ -946,7 +955,7 @@ LKL_API int lkl_mlp_get_params(lkl_mlp* n, float* out) {
 LKL_API int lkl_mlp_set_params(lkl_mlp* n, const float* in) {
     if (!n || !in) return -1;
     if (hipStreamSynchronize(n->stream) != hipSuccess) return -1;
-    if (hipMemcpy(n->p, in, NPARAM * 4, hipMemcpyHostToDevice) != hipSuccess) return -1;
+    if (h2d(n->p, in, NPARAM * 4, n->stream) != hipSuccess) return -1;
     hipLaunchKernelGGL(to_bf16_kernel, dim3(1024), dim3(256), 0, n->stream, n->p, n->p16, NPARAM);
     refresh_transposed(n, n->stream);
     return hipStreamSynchronize(n->stream) == hipSuccess ? 0 : -1;
@@ -1066,7 +1075,7 @@ LKL_API int lkl_mlp_forward_host(lkl_mlp* n, const unsigned short* x, int B, flo
     float* dy = nullptr;
     int rc = 0;
     if (hipMalloc(&dx, (size_t)B * NF * 2) || hipMalloc((void**)&dy, (size_t)B * 4)) rc = -1;
-    if (!rc && hipMemcpy(dx, x, (size_t)B * NF * 2, hipMemcpyHostToDevice)) rc = -1;
+    if (!rc && h2d(dx, x, (size_t)B * NF * 2, n->stream)) rc = -1;
     if (!rc) rc = lkl_mlp_forward(n, dx, B, dy, n->stream);
     if (!rc && hipStreamSynchronize(n->stream)) rc = -1;
     if (!rc && hipMemcpy(y, dy, (size_t)B * 4, hipMemcpyDeviceToHost)) rc = -1;
@@ -1083,8 +1092,8 @@ LKL_API int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* 
     float* dt = nullptr;
     int rc = 0;
     if (hipMalloc(&dx, (size_t)B * NF * 2) || hipMalloc((void**)&dt, (size_t)B * 4)) rc = -1;
-    if (!rc && (hipMemcpy(dx, x, (size_t)B * NF * 2, hipMemcpyHostToDevice) ||
-                hipMemcpy(dt, t, (size_t)B * 4, hipMemcpyHostToDevice))) rc = -1;
+    if (!rc && (h2d(dx, x, (size_t)B * NF * 2, n->stream) ||
+                h2d(dt, t, (size_t)B * 4, n->stream))) rc = -1;
     if (!rc) rc = lkl_mlp_grad(n, dx, dt, B, loss);
     if (hipStreamSynchronize(n->stream) != hipSuccess) rc = -1;  // before the inputs are freed
     if (dx) (void)hipFree(dx);
