@@ -74,11 +74,6 @@ struct DevComplex {
     const float4* rrec;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
-    // work-ordered tiles over lists: each atom's list is cut into segments of
-    // at most seg entries (a multiple of LIST_PAD; 0 = one item per atom), at
-    // most maxseg per atom (sizes the tile's item array)
-    unsigned seg;
-    int maxseg;
     // ligand, spatially sorted (Morton order of the reference coordinates)
     const float4* lxyzq;
     const float4* lpar;
@@ -90,6 +85,8 @@ struct DevComplex {
     const float* phi;
     int nx, ny, nz;
     double ox, oy, oz, sp;
+    double inv_sp;  // 1 / sp, exact when sp_pow2
+    int sp_pow2;    // sp is a power of two: x / sp == x * inv_sp
     // corner-packed copy (NULL = sample phi): per cell (ix, iy, iz) < (nx-1,
     // ny-1, nz-1), x fastest, two float4 {c000, c100, c010, c110} and
     // {c001, c101, c011, c111} -- the cell's 8 corners in one 32-byte read
@@ -140,27 +137,38 @@ __device__ __forceinline__ double exact_d2(double x, double y, double z, float4 
 }
 
 // grid_dna.rs:159-197 — f64 trilinear interpolation, offsets rounded to f32.
+// Branch-free: outside the grid (or for NaN input) the corner reads go to cell
+// 0 and the result is selected to 0, so the loads of several samples can be
+// issued together.  With a power-of-two spacing the division is a
+// multiplication by the exact reciprocal (the same IEEE result).
 __device__ __forceinline__ double sample_phi(const DevComplex& C, double x, double y, double z) {
 #pragma clang fp contract(off)
-    const double fx = (x - C.ox) / C.sp;
-    const double fy = (y - C.oy) / C.sp;
-    const double fz = (z - C.oz) / C.sp;
+    double fx, fy, fz;
+    if (C.sp_pow2) {
+        fx = (x - C.ox) * C.inv_sp;
+        fy = (y - C.oy) * C.inv_sp;
+        fz = (z - C.oz) * C.inv_sp;
+    } else {
+        fx = (x - C.ox) / C.sp;
+        fy = (y - C.oy) / C.sp;
+        fz = (z - C.oz) / C.sp;
+    }
     const double ix = floor(fx), iy = floor(fy), iz = floor(fz);
-    if (ix < 0.0 || iy < 0.0 || iz < 0.0) return 0.0;
-    if (ix + 1.0 >= (double)C.nx || iy + 1.0 >= (double)C.ny || iz + 1.0 >= (double)C.nz)
-        return 0.0;
-    const double tx = (double)(float)(fx - ix);
-    const double ty = (double)(float)(fy - iy);
-    const double tz = (double)(float)(fz - iz);
+    const bool in = ix >= 0.0 && iy >= 0.0 && iz >= 0.0 && ix + 1.0 < (double)C.nx &&
+                    iy + 1.0 < (double)C.ny && iz + 1.0 < (double)C.nz;
+    const size_t jx = in ? (size_t)ix : 0, jy = in ? (size_t)iy : 0, jz = in ? (size_t)iz : 0;
+    const double tx = in ? (double)(float)(fx - ix) : 0.0;
+    const double ty = in ? (double)(float)(fy - iy) : 0.0;
+    const double tz = in ? (double)(float)(fz - iz) : 0.0;
     double c000, c100, c010, c110, c001, c101, c011, c111;
     if (C.phi8) {
-        const size_t c = ((size_t)iz * (size_t)(C.ny - 1) + (size_t)iy) * (size_t)(C.nx - 1) + (size_t)ix;
+        const size_t c = (jz * (size_t)(C.ny - 1) + jy) * (size_t)(C.nx - 1) + jx;
         const float4 lo = C.phi8[2 * c], hi = C.phi8[2 * c + 1];
         c000 = lo.x; c100 = lo.y; c010 = lo.z; c110 = lo.w;
         c001 = hi.x; c101 = hi.y; c011 = hi.z; c111 = hi.w;
     } else {
         const size_t sy = (size_t)C.nx, sz = (size_t)C.nx * (size_t)C.ny;
-        const size_t b = (size_t)iz * sz + (size_t)iy * sy + (size_t)ix;
+        const size_t b = jz * sz + jy * sy + jx;
         const float* __restrict__ g = C.phi;
         c000 = g[b]; c100 = g[b + 1];
         c010 = g[b + sy]; c110 = g[b + sy + 1];
@@ -173,7 +181,8 @@ __device__ __forceinline__ double sample_phi(const DevComplex& C, double x, doub
     const double c11 = c011 * (1.0 - tx) + c111 * tx;
     const double c0 = c00 * (1.0 - ty) + c10 * ty;
     const double c1 = c01 * (1.0 - ty) + c11 * ty;
-    return c0 * (1.0 - tz) + c1 * tz;
+    const double r = c0 * (1.0 - tz) + c1 * tz;
+    return in ? r : 0.0;
 }
 
 // Monotone f32 key of an f64 squared distance that keeps membership of the

@@ -275,7 +275,7 @@ __device__ __forceinline__ void near_any(const DevComplex& C, double x64, double
 // in its window.  Used to order each tile's ligand atoms by expected work and
 // to skip those with none; it never changes a sum.
 constexpr float WORK_H = 2.0f, WORK_PAD = 12.0f;
-constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (48 KB of LDS lists)
+constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (LDS arrays)
 
 __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz,
                                 int nx, int ny, long long nodes, float r2, unsigned short* __restrict__ out) {
@@ -474,103 +474,135 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
     __syncthreads();  // the LDS lists may be reused by the caller
 }
 
-// Work-ordered near field over neighbour lists with list segments as the work
-// items: pass 1 transforms every atom of [j0, j1), adds its far field, looks up
-// its list [b, e) and counts its segments (at most C.seg entries each) into 16
-// buckets by descending segment length; the items are then scattered in bucket
-// order and pass 2 walks one segment per item, so a long list is shared by
-// several lanes and the lanes of a wavefront walk segments of similar length.
-// With desolvation, each item's ligand-side minimum meets the atom's in LDS
-// (64-bit atomic min on the f64 bits, exact) and pass 3 adds the per-atom
-// terms.  All threads of the block must call it (barriers).  LDS (8-byte
-// aligned): [minL u64 x nT if DES][b u32 x nT][e u32 x nT][items u32 x
-// nT * C.maxseg].
-constexpr int SEG_BKT = 16;
+// Near field over neighbour lists, lanes grouped for coalescing: pass 1
+// transforms every atom of [j0, j1), adds its far field and looks up its list
+// [b, e) (P1 atoms per thread per round, loads issued together); the atoms
+// with a non-empty list are then compacted in their (Morton) order, and the
+// wavefronts take groups of GA consecutive active atoms from a shared counter.
+// Within a wavefront each GL-lane subgroup walks one atom's list, lane r taking
+// entries r, r + GL, ..., r + (GK-1) GL of each GK*GL-entry step: the index
+// loads of a subgroup are contiguous, the records of neighbouring atoms'
+// lists (which overlap) share cache lines, and a lane keeps GK gathers in
+// flight.  With desolvation the ligand-side minimum of an atom is reduced over
+// its subgroup by shuffles (exact) and the per-atom term is added once.  All
+// threads of the block must call it (barriers).  LDS: [b u32 x nT][e u32 x
+// nT][active u16 x nT].
+constexpr int GL = 4;        // lanes per list
+constexpr int GK = 8;        // entries per lane per step
+constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 
 template <bool FAR, bool ELEC, bool CLASH, int DES>
-__device__ __forceinline__ void near_segments(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
-                                              unsigned* __restrict__ minR, long long& e, long long& v,
-                                              long long& s) {
-    __shared__ unsigned bcnt[SEG_BKT], boff[SEG_BKT];
+__device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
+                                            unsigned* __restrict__ minR, long long& e, long long& v,
+                                            long long& s) {
+    __shared__ unsigned wtot[16];
+    __shared__ unsigned next_group;
     const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+    const int lane = tid & 63, w = tid >> 6, nw = (bs + 63) >> 6;
     const int nT = j1 - j0;
-    const unsigned seg = C.seg;
-    unsigned long long* minL = reinterpret_cast<unsigned long long*>(lds);
-    unsigned* ab = reinterpret_cast<unsigned*>(DES != DES_NONE ? (void*)(minL + nT) : lds);
+    unsigned* ab = reinterpret_cast<unsigned*>(lds);
     unsigned* ae = ab + nT;
-    unsigned* items = ae + nT;
-    // bucket of a segment of n entries (1 <= n <= seg): 0 for full, larger for shorter
-    auto seg_bucket = [&](unsigned n) -> int {
-        return (int)(((seg - n) * (unsigned)SEG_BKT) / (seg + 1u));
-    };
-    for (int k = tid; k < SEG_BKT; k += bs) bcnt[k] = 0u;
-    __syncthreads();
-    for (int t = tid; t < nT; t += bs) {
-        const float4 lb = C.lxyzq[j0 + t];
-        double x, y, z;
-        transform(M, lb.x, lb.y, lb.z, x, y, z);
-        if (FAR && lb.w != 0.0f) e += to_fix((double)lb.w * sample_phi(C, x, y, z), E_SCALE);
-        const int n = list_node(C, x, y, z);
-        unsigned b = 0u, en = 0u;
-        if (n >= 0) {
-            b = C.noff[n];
-            en = C.noff[n + 1];
+    unsigned short* act = reinterpret_cast<unsigned short*>(ae + nT);
+    if (tid == 0) next_group = 0u;
+    constexpr int P1 = 3;
+    for (int t0 = tid; t0 < nT; t0 += P1 * bs) {
+        double px[P1], py[P1], pz[P1], ph[P1];
+        float q[P1];
+        unsigned nb[P1], ne[P1];
+#pragma unroll
+        for (int u = 0; u < P1; ++u) {
+            const int t = t0 + u * bs;
+            const float4 lb = C.lxyzq[j0 + (t < nT ? t : t0)];
+            transform(M, lb.x, lb.y, lb.z, px[u], py[u], pz[u]);
+            q[u] = lb.w;
         }
-        ab[t] = b;
-        ae[t] = en;
-        if (DES != DES_NONE) minL[t] = (unsigned long long)__double_as_longlong(MIN_INIT);
-        const unsigned len = en - b;
-        if (len) {
-            const unsigned full = len / seg, rest = len % seg;
-            if (full) atomicAdd(&bcnt[0], full);
-            if (rest) atomicAdd(&bcnt[seg_bucket(rest)], 1u);
+#pragma unroll
+        for (int u = 0; u < P1; ++u) {
+            ph[u] = FAR ? sample_phi(C, px[u], py[u], pz[u]) : 0.0;
+            const int n = list_node(C, px[u], py[u], pz[u]);
+            const int m = n < 0 ? 0 : n;
+            const unsigned b0 = C.noff[m], b1 = C.noff[m + 1];
+            nb[u] = n < 0 ? 0u : b0;
+            ne[u] = n < 0 ? 0u : b1;
+        }
+#pragma unroll
+        for (int u = 0; u < P1; ++u) {
+            const int t = t0 + u * bs;
+            if (t >= nT) break;
+            if (FAR && q[u] != 0.0f) e += to_fix((double)q[u] * ph[u], E_SCALE);
+            ab[t] = nb[u];
+            ae[t] = ne[u];
+            // no receptor atom in reach: the desolvation minimum stays at its initial value
+            if (DES != DES_NONE && ne[u] == nb[u])
+                s += to_fix(solv_term(MIN_INIT, C.lasa[j0 + t], C.ldes[j0 + t]), E_SCALE);
         }
     }
     __syncthreads();
-    if (tid == 0) {
-        unsigned acc0 = 0u;
-        for (int k = 0; k < SEG_BKT; ++k) {
-            boff[k] = acc0;
-            acc0 += bcnt[k];
+    // stable compaction of the active atoms (their order is kept)
+    unsigned nact = 0;
+    for (int c0 = 0; c0 < nT; c0 += bs) {
+        const int t = c0 + tid;
+        const bool a = t < nT && ae[t] != ab[t];
+        const unsigned long long m = __ballot(a);
+        if (lane == 0) wtot[w] = (unsigned)__popcll(m);
+        __syncthreads();
+        unsigned off = nact, tot = 0;
+        for (int q = 0; q < nw; ++q) {
+            if (q < w) off += wtot[q];
+            tot += wtot[q];
         }
-        bcnt[0] = acc0;  // total items (bcnt is not read again)
+        if (a) act[off + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)t;
+        nact += tot;
+        __syncthreads();
     }
-    __syncthreads();
-    for (int t = tid; t < nT; t += bs) {
-        const unsigned len = ae[t] - ab[t];
-        const unsigned full = len / seg, rest = len % seg;
-        if (full) {
-            const unsigned o = atomicAdd(&boff[0], full);
-            for (unsigned q = 0; q < full; ++q) items[o + q] = (unsigned)t | (q << 16);
-        }
-        if (rest) items[atomicAdd(&boff[seg_bucket(rest)], 1u)] = (unsigned)t | (full << 16);
-    }
-    __syncthreads();
-    const int nitems = (int)bcnt[0];
-    for (int k = tid; k < nitems; k += bs) {
-        const unsigned it = items[k];
-        const int t = (int)(it & 0xffffu);
-        const unsigned q = it >> 16;
+    const unsigned ngroups = (nact + (unsigned)GA - 1u) / (unsigned)GA;
+    const int sub = lane / GL, r = lane % GL;
+    const unsigned short dn = (unsigned short)C.nr;  // the dummy record
+    for (;;) {
+        unsigned g = 0;
+        if (lane == 0) g = atomicAdd(&next_group, 1u);
+        g = __shfl(g, 0, 64);
+        if (g >= ngroups) break;
+        const unsigned ai = g * (unsigned)GA + (unsigned)sub;
+        const bool has = ai < nact;
+        const int t = act[has ? ai : g * (unsigned)GA];
         const int j = j0 + t;
         const float4 lb = C.lxyzq[j];
         const float4 lp = C.lpar[j];
         double x, y, z;
         transform(M, lb.x, lb.y, lb.z, x, y, z);
-        const unsigned b = ab[t] + q * seg;
-        const unsigned en = min(b + seg, ae[t]);
-        double mloc = MIN_INIT;
         const bool lf = (DES != DES_NONE) ? (C.lflag[j] != 0) : false;
-        near_list<ELEC, CLASH, DES>(C, x, y, z, lb.w, lp.x, lp.y, lp.z != 0.0f, lf, b, en, e, v, mloc, minR);
-        if (DES != DES_NONE && mloc < MIN_INIT)
-            atomicMin(&minL[t], (unsigned long long)__double_as_longlong(mloc));
+        const unsigned b = has ? ab[t] : 0u, en = has ? ae[t] : 0u;
+        const float qjs = lb.w * E_SCALE_F, svjs = lp.x * V_SCALE_F, rj = lp.y;
+        const bool hj = lp.z != 0.0f;
+        const float xh = (float)x, yh = (float)y, zh = (float)z;
+        const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
+        double mloc = MIN_INIT;
+        for (unsigned p = b; __any(p < en); p += (unsigned)(GK * GL)) {
+            int id[GK];
+#pragma unroll
+            for (int k = 0; k < GK; ++k) {
+                const unsigned pos = p + (unsigned)(k * GL + r);
+                id[k] = pos < en ? (int)C.nidx[pos] : (int)dn;
+            }
+            float4 rq[GK], rp[GK];
+#pragma unroll
+            for (int k = 0; k < GK; ++k) {
+                rq[k] = C.rrec[2 * id[k]];
+                rp[k] = C.rrec[2 * id[k] + 1];
+            }
+#pragma unroll
+            for (int k = 0; k < GK; ++k)
+                pair_term<ELEC, CLASH, DES, true>(C, rq[k], id[k], x, y, z, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
+                                                  hj, lf, e, v, mloc, minR, rp[k]);
+        }
+        if (DES != DES_NONE) {
+#pragma unroll
+            for (int o = GL / 2; o > 0; o >>= 1) mloc = fmin(mloc, __shfl_xor(mloc, o, 64));
+            if (has && r == 0) s += to_fix(solv_term(mloc, C.lasa[j], C.ldes[j]), E_SCALE);
+        }
     }
-    __syncthreads();
-    if (DES != DES_NONE) {
-        for (int t = tid; t < nT; t += bs)
-            s += to_fix(solv_term(__longlong_as_double((long long)minL[t]), C.lasa[j0 + t], C.ldes[j0 + t]),
-                        E_SCALE);
-        __syncthreads();
-    }
+    __syncthreads();  // the LDS arrays may be reused by the caller
 }
 
 // One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
@@ -614,10 +646,10 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         // lists after the receptor minima when those live in LDS
         void* lists = DES == DES_LDS ? (void*)(((uintptr_t)(lds_min + C.nr) + 7u) & ~(uintptr_t)7u)
                                      : (void*)lds_min;
-        if (LIST && C.seg)
-            near_segments<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+        if constexpr (LIST)
+            near_groups<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
         else
-            near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+            near_sorted<false, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
     } else if constexpr (DES == DES_NONE && !PERATOM) {
         {
             for (int j = j0 + tid; j < j1; j += bs) {
@@ -704,10 +736,10 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
                                           long long& e, long long& v) {
     if (C.wgrid) {
         long long s_unused = 0;
-        if (LIST && C.seg)
-            near_segments<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+        if constexpr (LIST)
+            near_groups<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         else
-            near_sorted<LIST, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+            near_sorted<false, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
     }
     for (int j = j0 + (int)threadIdx.x; j < j1; j += (int)blockDim.x) {
@@ -832,7 +864,6 @@ struct lkl_ctx {
     int nn[3] = {0, 0, 0};
     float no[3] = {0.f, 0.f, 0.f};
     size_t list_entries = 0;  // 0 = no lists (row walk)
-    unsigned list_max = 0;    // longest (padded) list
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -951,7 +982,6 @@ static int build_lists(lkl_ctx* c) {
     if (c->d_noff) { (void)hipFree(c->d_noff); c->d_noff = nullptr; }
     if (c->d_nidx) { (void)hipFree(c->d_nidx); c->d_nidx = nullptr; }
     c->list_entries = 0;
-    c->list_max = 0;
     const int nr = c->nr;
     if (!(c->list_h > 0.0f) || nr <= 0 || nr > 65535) return 0;
     const float h = c->list_h;
@@ -980,10 +1010,8 @@ static int build_lists(lkl_ctx* c) {
     std::vector<unsigned> off((size_t)nodes + 1);
     HIPCK("readback", hipMemcpy(off.data(), c->d_noff, sizeof(unsigned) * (size_t)nodes, hipMemcpyDeviceToHost));
     unsigned long long acc = 0;
-    unsigned lmax = 0;
     for (long long k = 0; k < nodes; ++k) {
         const unsigned cnt = off[k];
-        lmax = std::max(lmax, cnt);
         off[k] = (unsigned)acc;
         acc += cnt;
         if (acc > (unsigned long long)LIST_ENTRIES_MAX) break;
@@ -1002,7 +1030,6 @@ static int build_lists(lkl_ctx* c) {
     HIPCK("sync", hipStreamSynchronize(c->stream));
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
     c->list_entries = (size_t)std::max(acc, 1ull);
-    c->list_max = lmax;
     c->version++;
     return 0;
 }
@@ -1222,6 +1249,11 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.oy = c->oy;
     C.oz = c->oz;
     C.sp = c->sp;
+    {
+        int ex = 0;
+        C.sp_pow2 = c->sp > 0.0 && std::frexp(c->sp, &ex) == 0.5 ? 1 : 0;
+        C.inv_sp = C.sp_pow2 ? std::ldexp(1.0, 1 - ex) : 1.0 / c->sp;
+    }
     return C;
 }
 
@@ -1336,26 +1368,10 @@ static int validate_flags(const lkl_ctx* c, unsigned flags) {
     return 0;
 }
 
-// Segment plan of a work-ordered list tile of nT ligand atoms: the shortest
-// segment length (from 64 entries, doubling) whose item array fits `budget`
-// bytes of LDS together with the per-atom arrays (near_segments' layout).
-// Returns the bytes needed, or 0 when no length fits (one item per atom then).
-constexpr unsigned SEG_MIN = 64;
-constexpr size_t TILE_LDS_BUDGET = 53 * 1024;  // three 512-thread tiles per CU
-
-static size_t plan_segments(size_t nT, unsigned list_max, bool des, size_t budget, unsigned* seg, int* maxseg) {
-    const unsigned lm = std::max(list_max, (unsigned)LIST_PAD);
-    for (unsigned sg = SEG_MIN;; sg *= 2) {
-        const size_t ms = (lm + sg - 1) / sg;
-        const size_t need = (des ? 8 * nT : 0) + 8 * nT + 4 * nT * ms + 16;
-        if (need <= budget) {
-            *seg = sg;
-            *maxseg = (int)ms;
-            return need;
-        }
-        if (sg >= lm) return 0;
-    }
-}
+// LDS bytes of a work-ordered tile of nT ligand atoms: near_groups' arrays
+// (list bounds and the active list) over neighbour lists, else near_sorted's
+// (index list and bucket ids).
+static size_t tile_lds(size_t nT, bool lists) { return lists ? 10 * nT + 16 : 3 * nT + 16; }
 
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
@@ -1367,7 +1383,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     size_t shm = 0;
     // work-ordered near field: tile-local lists of (chunk x 3) bytes in LDS
     const bool want_sort = C.wgrid != nullptr && !peratom;
-    const size_t list_nl = (size_t)c->nl * 3 + 16;
+    const bool list = C.noff != nullptr;
     if (flags & F_DESOLV) {
         if (!d_outS) { set_err("F_DESOLV requires an outS buffer"); return -1; }
         const size_t need = (size_t)c->nr * sizeof(unsigned);
@@ -1379,14 +1395,15 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
             des = DES_GLOBAL;
             if (dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
         }
-        // desolvation tiles hold the whole ligand: sort only if its list fits
-        if (want_sort && c->nl <= WORK_TILE_MAX && shm + list_nl + 1024 <= c->lds_max) shm += list_nl;
+        // desolvation tiles hold the whole ligand: order the near field only
+        // if its arrays fit (8 bytes of alignment after the receptor minima)
+        const size_t tl = tile_lds((size_t)c->nl, list) + 8;
+        if (want_sort && c->nl <= WORK_TILE_MAX && shm + tl + 1024 <= c->lds_max) shm += tl;
         else C.wgrid = nullptr;
     }
     if (peratom && dev_reserve(&c->d_pa_e, &c->cap_pa, (size_t)N * c->nl)) return -1;
     if (peratom && dev_reserve(&c->d_pa_v, &c->cap_pa_v, (size_t)N * c->nl)) return -1;
     score_fn fn;
-    const bool list = C.noff != nullptr;
     if (peratom) fn = list ? pick3<true, DES_NONE, true>(flags) : pick3<false, DES_NONE, true>(flags);
     else if (des == DES_LDS) fn = list ? pick3<true, DES_LDS, false>(flags) : pick3<false, DES_LDS, false>(flags);
     else if (des == DES_GLOBAL)
@@ -1394,19 +1411,11 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     else fn = list ? pick3<true, DES_NONE, false>(flags) : pick3<false, DES_NONE, false>(flags);
     int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
     if (des == DES_NONE && want_sort) {
-        if (chunk > WORK_TILE_MAX) chunk = WORK_TILE_MAX;  // LDS list bound
-        shm = (size_t)chunk * 3 + 16;                      // index list + bucket ids
+        const int tmax = (int)std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (list ? 10 : 3));
+        if (chunk > tmax) chunk = tmax;  // LDS bound of the tile's arrays
+        shm = tile_lds((size_t)chunk, list);
     }
     if (peratom) C.wgrid = nullptr;
-    if (C.wgrid && C.noff) {  // list segments as work items when their arrays fit
-        const size_t base = des == DES_LDS ? (size_t)c->nr * sizeof(unsigned) + 8 : 0;
-        const size_t budget = std::min(c->lds_max - std::min(c->lds_max, (size_t)1024), TILE_LDS_BUDGET);
-        if (base < budget) {
-            const size_t need = plan_segments((size_t)chunk, c->list_max, des != DES_NONE, budget - base, &C.seg,
-                                              &C.maxseg);
-            if (need) shm = base + need;
-        }
-    }
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
@@ -1731,19 +1740,19 @@ static int batch_refresh(lkl_multi* b) {
     std::vector<ProblemDesc> pd(b->K);
     int tile = 0;
     b->lds = 0;
-    b->list = true;
+    b->list = true;  // one kernel variant for the batch: lists unless a problem lacks them
+    for (int r = 0; r < b->K; ++r) {
+        cx[r] = dev_view(b->ctxs[b->order[r]]);
+        if (!cx[r].noff && b->ctxs[b->order[r]]->nr > 0) b->list = false;
+    }
     for (int r = 0; r < b->K; ++r) {
         const int k = b->order[r];
         lkl_ctx* c = b->ctxs[k];
-        cx[r] = dev_view(c);
-        if (!cx[r].noff && c->nr > 0) b->list = false;
         int chunk = b->chunk_mult > 0 ? std::min(c->nl, b->block * b->chunk_mult) : c->nl;
-        if (cx[r].wgrid) {
-            chunk = std::min(chunk, WORK_TILE_MAX);  // LDS list bound
-            size_t need = 0;
-            if (cx[r].noff)
-                need = plan_segments((size_t)chunk, c->list_max, false, TILE_LDS_BUDGET, &cx[r].seg, &cx[r].maxseg);
-            b->lds = std::max(b->lds, need ? need : (size_t)chunk * 3 + 16);
+        if (cx[r].wgrid) {  // LDS bound of the work-ordered tile's arrays
+            const size_t tmax = std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (b->list ? 10 : 3));
+            chunk = std::min(chunk, (int)tmax);
+            b->lds = std::max(b->lds, tile_lds((size_t)chunk, b->list));
         }
         pd[r].tile_begin = tile;
         pd[r].nchunks = (c->nl + chunk - 1) / chunk;
