@@ -488,8 +488,8 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 // threads of the block must call it (barriers).  LDS: [b u32 x nT][e u32 x
 // nT][active u16 x nT].
 constexpr int GL = 4;        // lanes per list
-constexpr int GK = 8;        // entries per lane per step
 constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
+constexpr int GK = 8;        // entries per lane per step
 
 template <bool FAR, bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
@@ -646,10 +646,12 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         // lists after the receptor minima when those live in LDS
         void* lists = DES == DES_LDS ? (void*)(((uintptr_t)(lds_min + C.nr) + 7u) & ~(uintptr_t)7u)
                                      : (void*)lds_min;
-        if constexpr (LIST)
+        // desolvation keeps one atom per lane: grouped lanes would contend
+        // for the same receptor minima
+        if constexpr (LIST && DES == DES_NONE)
             near_groups<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
         else
-            near_sorted<false, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+            near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
     } else if constexpr (DES == DES_NONE && !PERATOM) {
         {
             for (int j = j0 + tid; j < j1; j += bs) {
@@ -1369,9 +1371,9 @@ static int validate_flags(const lkl_ctx* c, unsigned flags) {
 }
 
 // LDS bytes of a work-ordered tile of nT ligand atoms: near_groups' arrays
-// (list bounds and the active list) over neighbour lists, else near_sorted's
-// (index list and bucket ids).
-static size_t tile_lds(size_t nT, bool lists) { return lists ? 10 * nT + 16 : 3 * nT + 16; }
+// (list bounds and the active list) when grouped, else near_sorted's (index
+// list and bucket ids).
+static size_t tile_lds(size_t nT, bool grouped) { return grouped ? 10 * nT + 16 : 3 * nT + 16; }
 
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
@@ -1397,7 +1399,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         }
         // desolvation tiles hold the whole ligand: order the near field only
         // if its arrays fit (8 bytes of alignment after the receptor minima)
-        const size_t tl = tile_lds((size_t)c->nl, list) + 8;
+        const size_t tl = tile_lds((size_t)c->nl, false) + 8;
         if (want_sort && c->nl <= WORK_TILE_MAX && shm + tl + 1024 <= c->lds_max) shm += tl;
         else C.wgrid = nullptr;
     }
