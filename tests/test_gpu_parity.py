@@ -204,6 +204,28 @@ def test_list_build_counts(gpu):
         assert abs(got - want) <= 8, (h, got, want)  # a pair exactly on the f32 boundary may flip
     ctx.set_list_spacing(0.0)
     assert ctx.list_info()[0] == 0
+    with pytest.raises(lklight.GpuError, match="list spacing"):
+        ctx.set_list_spacing(0.1)
+    ctx.close()
+
+
+def test_large_receptor_row_walk(gpu):
+    """A 70,000-atom receptor exceeds the lists' 16-bit indices: the context
+    falls back to the cell-row walk and still matches the oracle (vdw, no far
+    field, 12 poses)."""
+    say("large receptor: 70,000 atoms")
+    ax = synth.scaled_axes(70000, (1.0, 0.95, 0.9))
+    rec, _ = synth.make_mol(70000, ax, 123, +10.0)
+    lax = synth.scaled_axes(300, (1.0, 1.0, 1.0))
+    lig, _ = synth.make_mol(300, lax, 124, -2.0, center=True)
+    cx = synth.Complex(rec=rec, lig=lig, meta={"rec_axes": ax})
+    ctx = lklight.Context(cx, with_field=False)
+    assert ctx.list_info()[0] == 0
+    poses = synth.make_poses(12, ax, 9)
+    got = run(ctx, poses, lklight.FLAGS_VDW)
+    orc = O.Oracle(cx)
+    assert_scores("vdw", got, orc.batch("vdw", poses, nthreads=8))
+    assert (got["v"] != 0).sum() >= 6  # most poses touch the receptor
     ctx.close()
 
 
