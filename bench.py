@@ -92,6 +92,8 @@ def main():
                          "c4: 64-complex batch; c5: surrogate-screened GSO vs plain exact GSO")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="c2: skip the surrogate-screened GSO comparison")
+    ap.add_argument("--no-gso", action="store_true",
+                    help="c2: skip the host-driven vs device-resident GSO step comparison")
     ap.add_argument("--sur-train", type=int, default=2_000_000, help="c5: training poses")
     ap.add_argument("--sur-epochs", type=int, default=6)
     ap.add_argument("--sur-swarms", type=int, default=40)
@@ -259,10 +261,52 @@ def main():
                      "n_near_per_pose": n_near, "n_far_per_pose": n_far},
         "cpu_baseline": cpu,
     }
+    if args.workload == "c2" and not args.no_gso:
+        out["gso_step"] = gso_step_compare(cx, ctx, args.family)
     ctx.close()
     if not args.no_surrogate:
         out["surrogate"] = surrogate_experiment(args)
     print(json.dumps(out))
+
+
+def gso_step_compare(cx, ctx, family, steps=50):
+    """One GSO step at the workload's size (6 swarms x 200 glowworms) driven
+    from the host (lkl_gso_update_luciferin + lkl_gso_movement_phase: a
+    blocking scoring call and the movement on the CPU) against the
+    device-resident step (lkl_gso_device_run: scoring, luciferin and movement
+    kernels on one stream, one call for all steps).  Both start from the same
+    state after 2 warm-up steps; their trajectories agree to the last bit of
+    slerp's acos/sin, so the moved counts are reported side by side."""
+    import numpy as np
+    from lklight import gso as G
+    from lklight import synth
+    sizes = [200] * 6
+    pos = synth.make_swarm_positions(6, 200, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
+    res = {}
+    for mode in ("host", "device"):
+        g = G.GSO(ctx, family, pos, seed=42, swarm_sizes=sizes)
+        if mode == "host":
+            g.run(2)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g.update_luciferin()
+                g.movement_phase()
+            dt = time.perf_counter() - t0
+        else:
+            g.device_run(2)
+            t0 = time.perf_counter()
+            g.device_run(steps)
+            dt = time.perf_counter() - t0
+        st = g.state()
+        res[mode] = {"ms_per_step": dt / steps * 1e3, "moved_last_step": int(st["moved"].sum()),
+                     "best_score": float(np.max(st["scoring"]))}
+        g.close()
+    res["speedup"] = res["host"]["ms_per_step"] / res["device"]["ms_per_step"]
+    res["glowworms"] = 1200
+    res["steps"] = steps
+    log("gso step: host-driven %.3f ms, device-resident %.3f ms (%.2fx)"
+        % (res["host"]["ms_per_step"], res["device"]["ms_per_step"], res["speedup"]))
+    return res
 
 
 def surrogate_experiment(args):

@@ -283,6 +283,18 @@ int lkl_gso_update_luciferin(lkl_gso* gso);
 int lkl_gso_movement_phase(lkl_gso* gso);
 /* GSO::run (lib.rs:66-78): `steps` x (update_luciferin, movement_phase). */
 int lkl_gso_run(lkl_gso* gso, int steps);
+/* GSO::run's steps with the glowworm state resident on the device (no
+ * reference counterpart; replaces the per-step host round trip of
+ * swarm.rs:70-174): each step is the scoring launch over all poses, the
+ * luciferin update and the movement as kernels on `stream` (NULL = a
+ * stream of the driver's own), with every swarm's StdRng draws made on the host in the
+ * reference order and uploaded once.  The movement's slerp uses correctly
+ * rounded acos/sin, so poses can differ from lkl_gso_run's (host libm) in the
+ * last bit; decisions (neighbours, moves) are the same unless a comparison
+ * sits within that rounding.  The state is copied back to the host at the end
+ * (neighbour ids are not kept, only their count).  Needs a GPU context; no
+ * fallback.  0 / -1. */
+int lkl_gso_device_run(lkl_gso* gso, int steps, void* stream);
 /* Number of glowworms over all swarms. */
 int lkl_gso_size(const lkl_gso* gso);
 /* Per glowworm (swarm order): pose (w, x, y, z, tx, ty, tz), luciferin, last
@@ -312,6 +324,12 @@ int lkl_gso_save(const lkl_gso* gso, int swarm, int step, const char* directory)
 void lkl_rng_f64s(unsigned long long seed, int n, double* out);
 /* Quaternion::slerp (qt.rs:80-104) of (w, x, y, z) quaternions. */
 void lkl_quat_slerp(const double* a, const double* b, double t, double* out);
+/* The device GSO step's math on the host (for tests): correctly rounded sin
+ * (0 <= x <= 1.6) and acos (0 <= d <= 1) by double-double evaluation, and
+ * the slerp of qt.rs:80-104 built on them. */
+double lkl_sin_cr(double x);
+double lkl_acos_cr(double d);
+void lkl_quat_slerp_cr(const double* a, const double* b, double t, double* out);
 
 /* ── learned surrogate (no reference counterpart: the screening step that
  * ranks candidate moves before exact scoring) ────────────────────────────────

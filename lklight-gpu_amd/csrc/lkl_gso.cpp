@@ -206,6 +206,14 @@ struct Swarm {
 
 typedef int (*lkl_score_fallback)(void* user, const double* poses, int n, double* scores);
 
+// Device-resident steps (lkl_gsod.hip).
+struct lkl_gsod_buf;
+void lkl_gsod_free(lkl_gsod_buf* d);
+int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, int n, int n_swarms,
+                 const int* swarm_off, double* pose7, double* luc, double* vis, double* score,
+                 unsigned char* moved, unsigned* step, int* nbc, const double* rnd, int steps, void* stream,
+                 char* err, size_t err_len);
+
 struct lkl_gso {
     lkl_ctx* ctx = nullptr;
     int family = 0;
@@ -215,6 +223,8 @@ struct lkl_gso {
     void* fallback_user = nullptr;
     long long gpu_launches = 0, fallback_calls = 0, scored = 0;
     std::string err;
+    lkl_gsod_buf* dev = nullptr;  // device-resident step buffers
+    ~lkl_gso() { lkl_gsod_free(dev); }
 };
 
 static unsigned family_flags(int family) {
@@ -317,6 +327,69 @@ LKL_API int lkl_gso_run(lkl_gso* G, int steps) {
         if (lkl_gso_update_luciferin(G)) return -1;
         lkl_gso_movement_phase(G);
     }
+    return 0;
+}
+
+// swarm.rs:70-174 for `steps` full steps (luciferin, then movement) with the
+// state on the device: see lkl_gsod.hip.  The swarms' StdRng draws are made
+// here, in the order the host movement makes them, and the state returns to
+// the host glowworms at the end (neighbour ids are not kept, only their count).
+LKL_API int lkl_gso_device_run(lkl_gso* G, int steps, void* stream) {
+    if (!G || steps < 0 || !G->ctx) return -1;
+    if (steps == 0) return 0;
+    int n = 0;
+    std::vector<int> off(1, 0);
+    for (const Swarm& sw : G->swarms) {
+        n += (int)sw.g.size();
+        off.push_back(n);
+    }
+    const size_t N = (size_t)n;
+    std::vector<double> pose(7 * N), luc(N), vis(N), score(N), rnd((size_t)steps * N);
+    std::vector<unsigned char> moved(N);
+    std::vector<unsigned> step(N);
+    std::vector<int> nbc(N);
+    size_t k = 0;
+    for (const Swarm& sw : G->swarms)
+        for (const Glowworm& w : sw.g) {
+            double* p = &pose[7 * k];
+            p[0] = w.q.w; p[1] = w.q.x; p[2] = w.q.y; p[3] = w.q.z;
+            p[4] = w.t[0]; p[5] = w.t[1]; p[6] = w.t[2];
+            luc[k] = w.luciferin;
+            vis[k] = w.vision;
+            score[k] = w.scoring;
+            moved[k] = w.moved ? 1 : 0;
+            step[k] = w.step;
+            ++k;
+        }
+    for (int s = 0; s < steps; ++s) {  // serial draws per swarm, id order (swarm.rs:165)
+        size_t b = (size_t)s * N;
+        for (Swarm& sw : G->swarms)
+            for (size_t i = 0; i < sw.g.size(); ++i) rnd[b++] = sw.rng.next_f64();
+    }
+    char err[256] = {0};
+    if (lkl_gsod_run(&G->dev, G->ctx, G->family, G->flags, n, (int)G->swarms.size(), off.data(), pose.data(),
+                     luc.data(), vis.data(), score.data(), moved.data(), step.data(), nbc.data(), rnd.data(), steps,
+                     stream, err, sizeof err)) {
+        G->err = err;
+        fprintf(stderr, "[lklight-gpu] %s\n", err);
+        return -1;
+    }
+    k = 0;
+    for (Swarm& sw : G->swarms)
+        for (Glowworm& w : sw.g) {
+            const double* p = &pose[7 * k];
+            w.q = {p[0], p[1], p[2], p[3]};
+            w.t = {p[4], p[5], p[6]};
+            w.luciferin = luc[k];
+            w.vision = vis[k];
+            w.scoring = score[k];
+            w.moved = moved[k] != 0;
+            w.step = step[k];
+            w.nb.assign((size_t)nbc[k], -1);
+            ++k;
+        }
+    G->gpu_launches += steps;
+    G->scored += (long long)steps * n;
     return 0;
 }
 

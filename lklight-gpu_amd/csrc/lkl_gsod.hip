@@ -1,0 +1,221 @@
+// lkl_gsod.hip — device-resident GSO steps.
+//
+// The state of every glowworm of every swarm lives in device memory and one
+// step is three launches on one stream: the scoring kernel over all poses,
+// the luciferin update (swarm.rs:70-103: the moved glowworms take their new
+// score, luciferin = (1 - rho) l + gamma s), and the movement
+// (swarm.rs:105-174, glowworm.rs:84-166): per glowworm, the brighter
+// glowworms within its vision range in id order, their probabilities, the
+// choice by the step's uniform draw, the move towards the chosen neighbour
+// and the vision update.  The draws come from each swarm's StdRng on the host
+// (one per glowworm in id order, as swarm.rs:165), uploaded once for all steps.
+// Arithmetic is the host driver's in the same order with contraction off;
+// slerp's acos and sin are correctly rounded (lkl_gsomath.hpp).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/lklight_gpu.h"
+#include "lkl_gsomath.hpp"
+
+#define LKL_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// glowworm.rs:45-51, constants.rs:5-8
+constexpr double kRho = 0.5, kGamma = 0.4, kBeta = 0.08, kMaxVision = 5.0;
+constexpr int kMaxNeighbors = 5;
+constexpr double kTransStep = 0.5, kRotStep = 0.5;
+
+struct GsoView {
+    double* pose;     // n x 7 (w, x, y, z, tx, ty, tz), current
+    double* pose_nx;  // n x 7, written by the movement
+    double* luc;
+    double* vis;
+    double* score;
+    unsigned char* moved;
+    int* nbc;
+    unsigned* step;
+    const int* swarm_of;   // glowworm -> swarm
+    const int* swarm_off;  // swarm -> first glowworm (n_swarms + 1)
+    int n;
+};
+
+// gpu_score.rs:326-331 / gpu_family.rs:283-288; cpydock.rs:789-791.
+__device__ __forceinline__ double fold(int family, double e, double v, double s) {
+#pragma clang fp contract(off)
+    const double ek = e * 332.0 / 4.0;
+    return family == 3 ? -(ek + 0.1 * v) + s : -(ek + v);
+}
+
+__global__ void luciferin_kernel(GsoView G, const double* __restrict__ out, const double* __restrict__ outS,
+                                 int family) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    if (G.moved[i] || G.step[i] == 0u) G.score[i] = fold(family, out[2 * i], out[2 * i + 1], outS ? outS[i] : 0.0);
+    G.luc[i] = (1.0 - kRho) * G.luc[i] + kGamma * G.score[i];
+    G.step[i] += 1u;
+}
+
+// One glowworm per thread.  Reads the step's snapshot (pose, luc, vis of all
+// glowworms of the swarm) and writes only its own next pose, moved flag,
+// neighbour count and vision.
+__global__ void move_kernel(GsoView G, const double* __restrict__ rnd) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= G.n) return;
+    const int s = G.swarm_of[i], b = G.swarm_off[s], e = G.swarm_off[s + 1];
+    const double* pi = G.pose + 7 * (size_t)i;
+    const double li = G.luc[i], vi = G.vis[i], vr2 = vi * vi;
+    const double tx = pi[4], ty = pi[5], tz = pi[6];
+    int cnt = 0, first = -1;
+    double total = 0.0;
+    for (int k = b; k < e; ++k) {
+        if (k == i || !(li < G.luc[k])) continue;
+        const double* pk = G.pose + 7 * (size_t)k;
+        const double dx = tx - pk[4], dy = ty - pk[5], dz = tz - pk[6];
+        if (dx * dx + dy * dy + dz * dz < vr2) {
+            if (first < 0) first = k;
+            ++cnt;
+            total += G.luc[k] - li;
+        }
+    }
+    int nid = i;
+    if (cnt > 0) {  // glowworm.rs:114-126: walk the cumulative probabilities
+        const double r = rnd[i];
+        double sum = 0.0;
+        int last = -1;
+        for (int k = b; k < e && sum < r; ++k) {
+            if (k == i || !(li < G.luc[k])) continue;
+            const double* pk = G.pose + 7 * (size_t)k;
+            const double dx = tx - pk[4], dy = ty - pk[5], dz = tz - pk[6];
+            if (!(dx * dx + dy * dy + dz * dz < vr2)) continue;
+            double p = G.luc[k] - li;
+            p /= total;
+            sum += p;
+            last = k;
+        }
+        nid = last >= 0 ? last : first;
+    }
+    double* po = G.pose_nx + 7 * (size_t)i;
+    if (nid != i) {  // glowworm.rs:128-150
+        const double* pn = G.pose + 7 * (size_t)nid;
+        const double dx = pn[4] - tx, dy = pn[5] - ty, dz = pn[6] - tz;
+        const double norm = sqrt(dx * dx + dy * dy + dz * dz);
+        const double coef = kTransStep / norm;
+        double t0 = tx, t1 = ty, t2 = tz;
+        t0 += dx * coef;
+        t1 += dy * coef;
+        t2 += dz * coef;
+        double q[4];
+        lkl::gm::slerp_cr(pi, pn, kRotStep, q);
+        lkl::gm::normalize4(q[0], q[1], q[2], q[3]);
+        po[0] = q[0]; po[1] = q[1]; po[2] = q[2]; po[3] = q[3];
+        po[4] = t0; po[5] = t1; po[6] = t2;
+    } else {
+        for (int c = 0; c < 7; ++c) po[c] = pi[c];
+    }
+    G.moved[i] = nid != i ? 1 : 0;
+    G.nbc[i] = cnt;
+    G.vis[i] = fmin(kMaxVision, fmax(0.0, vi + kBeta * (double)(kMaxNeighbors - cnt)));
+}
+
+template <typename T>
+hipError_t grow(T** p, size_t* cap, size_t n) {
+    if (n <= *cap && *p) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    const hipError_t e = hipMalloc((void**)p, sizeof(T) * (n ? n : 1));
+    if (e == hipSuccess) *cap = n;
+    return e;
+}
+
+}  // namespace
+
+// Device buffers of one GSO driver (grown on demand, freed by lkl_gsod_free).
+struct lkl_gsod_buf {
+    double *pose = nullptr, *pose_nx = nullptr, *luc = nullptr, *vis = nullptr, *score = nullptr, *rnd = nullptr,
+           *out = nullptr, *outS = nullptr;
+    unsigned char* moved = nullptr;
+    int *nbc = nullptr, *swarm_of = nullptr, *swarm_off = nullptr;
+    unsigned* step = nullptr;
+    hipStream_t stream = nullptr;  // used when the caller passes none
+    size_t c_pose = 0, c_pose_nx = 0, c_luc = 0, c_vis = 0, c_score = 0, c_rnd = 0, c_out = 0, c_outS = 0,
+           c_moved = 0, c_nbc = 0, c_swarm_of = 0, c_swarm_off = 0, c_step = 0;
+};
+
+void lkl_gsod_free(lkl_gsod_buf* d) {
+    if (!d) return;
+    void* p[] = {d->pose, d->pose_nx, d->luc, d->vis, d->score, d->rnd, d->out, d->outS,
+                 d->moved, d->nbc, d->swarm_of, d->swarm_off, d->step};
+    for (void* q : p)
+        if (q) (void)hipFree(q);
+    if (d->stream) (void)hipStreamDestroy(d->stream);
+    delete d;
+}
+
+// Run `steps` device-resident steps.  Host arrays in and out (n glowworms,
+// the swarms contiguous): pose7 (w, x, y, z, tx, ty, tz), luciferin, vision,
+// score, moved, step, neighbour count (out only); rnd holds steps x n draws.
+// Returns 0, or -1 with a message in err.
+int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, int n, int n_swarms,
+                 const int* swarm_off, double* pose7, double* luc, double* vis, double* score,
+                 unsigned char* moved, unsigned* step, int* nbc, const double* rnd, int steps, void* stream,
+                 char* err, size_t err_len) {
+    auto fail = [&](const char* what, hipError_t e) {
+        snprintf(err, err_len, "lkl_gso_device_run: %s: %s", what, e == hipSuccess ? "failed" : hipGetErrorString(e));
+        return -1;
+    };
+    if (!*pd) *pd = new lkl_gsod_buf();
+    lkl_gsod_buf* d = *pd;
+    hipError_t e;
+    if (!stream && !d->stream) {
+        if ((e = hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+    }
+    // every upload, launch and readback of the run is ordered on this one stream
+    hipStream_t st = stream ? (hipStream_t)stream : d->stream;
+    const size_t N = (size_t)n;
+    std::vector<int> swarm_of(N);
+    for (int s = 0; s < n_swarms; ++s)
+        for (int k = swarm_off[s]; k < swarm_off[s + 1]; ++k) swarm_of[k] = s;
+#define GROW(field, count)                                                         \
+    if ((e = grow(&d->field, &d->c_##field, (count))) != hipSuccess) return fail("alloc", e);
+    GROW(pose, 7 * N) GROW(pose_nx, 7 * N) GROW(luc, N) GROW(vis, N) GROW(score, N) GROW(rnd, (size_t)steps * N)
+    GROW(out, 2 * N) GROW(outS, N) GROW(moved, N) GROW(nbc, N) GROW(swarm_of, N) GROW(swarm_off, (size_t)n_swarms + 1)
+    GROW(step, N)
+#undef GROW
+    struct Up { void* dst; const void* src; size_t bytes; };
+    const Up up[] = {{d->pose, pose7, 56 * N}, {d->luc, luc, 8 * N}, {d->vis, vis, 8 * N}, {d->score, score, 8 * N},
+                     {d->moved, moved, N}, {d->step, step, 4 * N}, {d->swarm_of, swarm_of.data(), 4 * N},
+                     {d->swarm_off, swarm_off, 4 * ((size_t)n_swarms + 1)}, {d->rnd, rnd, 8 * (size_t)steps * N}};
+    for (const Up& u : up)
+        if ((e = hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, st)) != hipSuccess) return fail("upload", e);
+    GsoView G{d->pose, d->pose_nx, d->luc, d->vis, d->score, d->moved, d->nbc, d->step, d->swarm_of, d->swarm_off, n};
+    const dim3 grid((unsigned)((N + 127) / 128)), block(128);
+    for (int k = 0; k < steps; ++k) {
+        if (lkl_ctx_score_async(ctx, G.pose, n, flags, d->out, (flags & LKL_F_DESOLV) ? d->outS : nullptr, st))
+            return fail("score", hipSuccess);
+        hipLaunchKernelGGL(luciferin_kernel, grid, block, 0, st, G, d->out,
+                           (flags & LKL_F_DESOLV) ? d->outS : nullptr, family);
+        hipLaunchKernelGGL(move_kernel, grid, block, 0, st, G, d->rnd + (size_t)k * N);
+        if ((e = hipGetLastError()) != hipSuccess) return fail("launch", e);
+        std::swap(G.pose, G.pose_nx);
+    }
+    struct Down { void* dst; const void* src; size_t bytes; };
+    const Down down[] = {{pose7, G.pose, 56 * N}, {luc, d->luc, 8 * N}, {vis, d->vis, 8 * N}, {score, d->score, 8 * N},
+                         {moved, d->moved, N}, {step, d->step, 4 * N}, {nbc, d->nbc, 4 * N}};
+    for (const Down& u : down)
+        if ((e = hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyDeviceToHost, st)) != hipSuccess) return fail("readback", e);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail("sync", e);
+    return 0;
+}
+
+// The device step's correctly rounded sin and acos, on the host (for tests).
+LKL_API double lkl_sin_cr(double x) { return lkl::gm::sin_cr(x); }
+LKL_API double lkl_acos_cr(double d) { return lkl::gm::acos_cr(d); }
+LKL_API void lkl_quat_slerp_cr(const double* a, const double* b, double t, double* out) {
+    lkl::gm::slerp_cr(a, b, t, out);
+}

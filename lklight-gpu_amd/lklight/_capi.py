@@ -118,6 +118,7 @@ def lib():
         "lkl_gso_update_luciferin": (C.c_int, [C.c_void_p]),
         "lkl_gso_movement_phase": (C.c_int, [C.c_void_p]),
         "lkl_gso_run": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_gso_device_run": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
         "lkl_gso_size": (C.c_int, [C.c_void_p]),
         "lkl_gso_state": (C.c_int, [C.c_void_p, _d, _d, _d, _d, _i, _u8]),
         "lkl_gso_set_poses": (C.c_int, [C.c_void_p, _d, _u8]),
@@ -154,6 +155,9 @@ def lib():
                                         C.c_void_p, C.c_void_p, C.c_void_p]),
         "lkl_rng_f64s": (None, [C.c_uint64, C.c_int, _d]),
         "lkl_quat_slerp": (None, [_d, _d, C.c_double, _d]),
+        "lkl_quat_slerp_cr": (None, [_d, _d, C.c_double, _d]),
+        "lkl_sin_cr": (C.c_double, [C.c_double]),
+        "lkl_acos_cr": (C.c_double, [C.c_double]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -57,6 +57,11 @@ class GSO:
     def run(self, steps):
         _capi.check(lib().lkl_gso_run(self.h, int(steps)), "lkl_gso_run")
 
+    def device_run(self, steps, stream=None):
+        """lkl_gso_device_run: `steps` full steps with the state on the device."""
+        _capi.check(lib().lkl_gso_device_run(self.h, int(steps), C.c_void_p(stream) if stream else None),
+                    "lkl_gso_device_run")
+
     def state(self):
         n = self.n
         pose = np.zeros((n, 7))

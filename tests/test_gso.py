@@ -186,3 +186,51 @@ def test_commit_sets_pose_and_score(small_vdw):
     assert list(st["scoring"]) == [1.5, 0.0, 3.5, 0.0]
     assert not st["moved"].any()
     g.close()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("method", ["dna", "vdw"])
+def test_gso_device_run_matches_host_driver(gpu, method):
+    """Device-resident steps (lkl_gso_device_run) against the host driver from
+    the same start, 3 swarms x 40 glowworms on the configs[0] complex, 12 steps:
+    at every step the moves, neighbour counts and vision ranges are identical,
+    poses agree to 1e-9 A (the device slerp's acos/sin are correctly rounded,
+    the host libm's within one ulp) and scores / luciferin to 1e-7 relative.
+    One 8-step call equals 8 one-step calls bit for bit."""
+    cx = synth.make_config("c1")
+    if method != "vdw":
+        cx.build_field()
+    pos = synth.make_swarm_positions(3, 40, cx.meta["rec_axes"], 17, lig_radius=cx.meta["lig_axes"][0])
+    ctx = lklight.Context(cx, with_field=method != "vdw")
+    sizes = [40, 40, 40]
+    host = G.GSO(ctx, method, pos, seed=11, swarm_sizes=sizes)
+    dev = G.GSO(ctx, method, pos, seed=11, swarm_sizes=sizes)
+    moved_total = 0
+    for s in range(12):
+        host.update_luciferin()
+        host.movement_phase()
+        dev.device_run(1)
+        a, b = host.state(), dev.state()
+        assert np.array_equal(a["moved"], b["moved"]), s
+        assert np.array_equal(a["neighbors"], b["neighbors"]), s
+        assert np.array_equal(a["vision"], b["vision"]), s
+        assert np.abs(a["pose"] - b["pose"]).max() <= 1e-9, (s, np.abs(a["pose"] - b["pose"]).max())
+        assert _rel(b["scoring"], a["scoring"]) <= 1e-7, s
+        assert _rel(b["luciferin"], a["luciferin"]) <= 1e-7, s
+        moved_total += int(a["moved"].sum())
+    assert moved_total > 100  # the swarms do move
+    one = G.GSO(ctx, method, pos, seed=11, swarm_sizes=sizes)
+    one.device_run(8)
+    many = G.GSO(ctx, method, pos, seed=11, swarm_sizes=sizes)
+    for _ in range(8):
+        many.device_run(1)
+    x, y = one.state(), many.state()
+    for k in x:
+        assert np.array_equal(x[k], y[k]), k
+    for g in (host, dev, one, many):
+        g.close()
+    ctx.close()
