@@ -205,6 +205,17 @@ def main():
     value = N * K / wall
     log("timed %d steps: %.3f s wall, kernel avg %.3f ms" % (K, wall, avg_kernel_s * 1e3))
 
+    # The same steps through the blocking host-buffer call (poses H2D, kernel,
+    # scores D2H, synchronise): the PCIe-inclusive rate, reported beside
+    # `value` (which has the poses resident in HBM), never as it.
+    host_poses = [np.ascontiguousarray(poses[(W + k) * N:(W + k + 1) * N]) for k in range(min(K, 20))]
+    ctx.score(host_poses[0], flags)
+    t0 = time.perf_counter()
+    for hp in host_poses:
+        ctx.score(hp, flags)
+    host_io_rate = N * len(host_poses) / (time.perf_counter() - t0)
+    log("host buffers (H2D + kernel + D2H per call): %.0f pose-evals/s" % host_io_rate)
+
     # CPU baseline (oracle) on a bounded sample of the same poses; its pair
     # counts also give the algorithmic bytes of the roofline figure.
     threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
@@ -260,13 +271,49 @@ def main():
                      "kernel_ms": avg_kernel_s * 1e3, "bytes_per_pose": b_pose,
                      "n_near_per_pose": n_near, "n_far_per_pose": n_far},
         "cpu_baseline": cpu,
+        "host_io_pose_evals_per_s": host_io_rate,
     }
     if args.workload == "c2" and not args.no_gso:
         out["gso_step"] = gso_step_compare(cx, ctx, args.family)
+        out["restraints"] = interface_timing(cx, ctx, d_poses[W * N:(W + 1) * N], N, stream)
     ctx.close()
     if not args.no_surrogate:
         out["surrogate"] = surrogate_experiment(args)
     print(json.dumps(out))
+
+
+def interface_timing(cx, ctx, d_poses, N, stream, reps=20):
+    """Cost of the restraint / membrane interface pass (lkl_ctx_interface_async)
+    on one step's poses: 100 receptor residues of 8 atoms, 20 ligand residues
+    of 5 atoms and 200 membrane beads, chosen at random."""
+    import numpy as np
+    import torch
+    from lklight import _capi
+    rng = np.random.default_rng(1)
+    rec_res = [list(range(a, a + 8)) for a in rng.choice(np.arange(0, cx.rec.n - 8, 8), 100, replace=False)]
+    lig_res = [list(range(a, a + 5)) for a in rng.choice(np.arange(0, cx.lig.n - 5, 5), 20, replace=False)]
+    membrane = sorted(rng.choice(cx.rec.n, 200, replace=False).tolist())
+    ctx.set_interface(rec_res, lig_res, membrane)
+    d_cnt = torch.zeros((N, 3), dtype=torch.int32, device="cuda")
+    L = _capi.lib()
+
+    def launch():
+        _capi.check(L.lkl_ctx_interface_async(ctx.h, d_poses.data_ptr(), N, d_cnt.data_ptr(), stream.cuda_stream),
+                    "lkl_ctx_interface_async")
+    launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        launch()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    cnt = d_cnt.cpu().numpy()
+    ctx.set_interface()
+    log("interface pass: %.3f ms per %d poses" % (ms, N))
+    return {"interface_ms_per_launch": ms, "poses": N, "rec_residues": 100, "lig_residues": 20,
+            "membrane_beads": 200, "mean_satisfied_rec_residues": float(cnt[:, 0].mean())}
 
 
 def gso_step_compare(cx, ctx, family, steps=50):
@@ -304,6 +351,8 @@ def gso_step_compare(cx, ctx, family, steps=50):
     res["speedup"] = res["host"]["ms_per_step"] / res["device"]["ms_per_step"]
     res["glowworms"] = 1200
     res["steps"] = steps
+    # the configuration's run is 100 steps of the 6 swarms
+    res["wall_s_100_steps"] = {m: res[m]["ms_per_step"] * 0.1 for m in ("host", "device")}
     log("gso step: host-driven %.3f ms, device-resident %.3f ms (%.2fx)"
         % (res["host"]["ms_per_step"], res["device"]["ms_per_step"], res["speedup"]))
     return res

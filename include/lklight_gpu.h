@@ -178,6 +178,29 @@ int lkl_ctx_set_work_sort(lkl_ctx* ctx, int on);
  * src/gpu_score.rs:61-112 there). */
 int lkl_ctx_set_list_spacing(lkl_ctx* ctx, float spacing);
 
+/* Interface flags for restraints / membrane (dna.rs:885-995, cpydock.rs:509,
+ * 639, scoring.rs:83-110), which keep the reference's restraint runs off the
+ * GPU batch (dna.rs:1047-1056).  Residues are CSR over atom indices in the
+ * caller's (reference) order: residue k holds atoms res_atoms[res_off[k] ..
+ * res_off[k+1]); membrane beads are receptor atom indices.  Counts replace
+ * the previous data; all zero clears it.  0 / -1. */
+int lkl_ctx_set_interface(lkl_ctx* ctx, const int* rec_res_off, const int* rec_res_atoms, int n_rec_res,
+                          const int* lig_res_off, const int* lig_res_atoms, int n_lig_res,
+                          const int* membrane_atoms, int n_membrane);
+/* 1 when the context holds restraints or a membrane, else 0. */
+int lkl_ctx_has_interface(const lkl_ctx* ctx);
+/* Numbers of receptor residues, ligand residues and membrane beads set. */
+int lkl_ctx_interface_sizes(const lkl_ctx* ctx, int* n_rec_res, int* n_lig_res, int* n_membrane);
+/* Per pose: counts[3k] = receptor residues with an atom at the interface
+ * (f64 squared distance <= 3.9^2 to some ligand atom), counts[3k+1] = the
+ * same for ligand residues, counts[3k+2] = membrane beads at the interface.
+ * The score fold is then score += (r/n_rec_res)*score + (l/n_lig_res)*score
+ * - 999*(m/n_membrane) (a fraction with no residues counts 0; the penalty only
+ * when m > 0), dna.rs:989-995.  Needs the neighbour lists.  Blocking on host
+ * buffers, or async on device buffers (`stream` NULL = the context's). */
+int lkl_ctx_interface(lkl_ctx* ctx, const double* poses, int N, int* counts);
+int lkl_ctx_interface_async(lkl_ctx* ctx, const double* d_poses, int N, int* d_counts, void* stream);
+
 /* Far-field layout (default on): besides phi, keep a corner-packed copy (the
  * 8 corners of every grid cell together, 32 bytes per cell, 8x the memory of
  * phi) so a trilinear sample is one aligned read.  Samples are bitwise those
@@ -281,7 +304,10 @@ void lkl_gso_destroy(lkl_gso* gso);
 int lkl_gso_update_luciferin(lkl_gso* gso);
 /* Swarm::movement_phase (swarm.rs:105-174) for every swarm.  0 / -1. */
 int lkl_gso_movement_phase(lkl_gso* gso);
-/* GSO::run (lib.rs:66-78): `steps` x (update_luciferin, movement_phase). */
+/* With restraints or a membrane on the context (lkl_ctx_set_interface) the GSO
+ * scores carry the restraint fold of dna.rs:989-995 (interface counts from
+ * the same GPU pass); the reference keeps such runs off the batch path.
+ * GSO::run (lib.rs:66-78): `steps` x (update_luciferin, movement_phase). */
 int lkl_gso_run(lkl_gso* gso, int steps);
 /* GSO::run's steps with the glowworm state resident on the device (no
  * reference counterpart; replaces the per-step host round trip of

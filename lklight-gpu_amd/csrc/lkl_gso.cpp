@@ -212,7 +212,7 @@ void lkl_gsod_free(lkl_gsod_buf* d);
 int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, int n, int n_swarms,
                  const int* swarm_off, double* pose7, double* luc, double* vis, double* score,
                  unsigned char* moved, unsigned* step, int* nbc, const double* rnd, int steps, void* stream,
-                 char* err, size_t err_len);
+                 char* err, size_t err_len);  // restraints: from lkl_ctx_has_interface / _sizes
 
 struct lkl_gso {
     lkl_ctx* ctx = nullptr;
@@ -239,8 +239,21 @@ static unsigned family_flags(int family) {
 
 // gpu_score.rs:326-331 / gpu_family.rs:283-288; cpydock.rs:789-791.
 static double fold(int family, double e, double v, double s) {
+#pragma clang fp contract(off)
     const double ek = e * 332.0 / 4.0;
     return family == 3 ? -(ek + 0.1 * v) + s : -(ek + v);
+}
+
+// dna.rs:989-995 / cpydock.rs:545-549 with scoring.rs:83-110 fractions from
+// interface counts (satisfied residues of each side, membrane beads).
+static double restraint_fold(double score, const int* cnt, const int* sizes) {
+#pragma clang fp contract(off)
+    const double perc_r = sizes[0] ? (double)cnt[0] / (double)sizes[0] : 0.0;
+    const double perc_l = sizes[1] ? (double)cnt[1] / (double)sizes[1] : 0.0;
+    const double inter = sizes[2] ? (double)cnt[2] / (double)sizes[2] : 0.0;
+    const double penalty = inter > 0.0 ? 999.0 * inter : 0.0;
+    score += perc_r * score + perc_l * score - penalty;
+    return score;
 }
 
 // Score `poses` (n x 7) into `scores` (GPU first, then the fallback if any).
@@ -249,9 +262,17 @@ static int score_batch(lkl_gso* G, const std::vector<double>& poses, int n, std:
     if (n == 0) return 0;
     if (G->ctx) {
         std::vector<double> out((size_t)n * 2), s((size_t)n);
+        const bool iface = lkl_ctx_has_interface(G->ctx) == 1;
+        std::vector<int> cnt(iface ? (size_t)n * 3 : 0);
+        int sizes[3] = {0, 0, 0};
         if (lkl_ctx_score(G->ctx, poses.data(), n, G->flags, out.data(),
-                          (G->flags & LKL_F_DESOLV) ? s.data() : nullptr) == 0) {
-            for (int k = 0; k < n; ++k) scores[k] = fold(G->family, out[2 * k], out[2 * k + 1], s[k]);
+                          (G->flags & LKL_F_DESOLV) ? s.data() : nullptr) == 0 &&
+            (!iface || (lkl_ctx_interface(G->ctx, poses.data(), n, cnt.data()) == 0 &&
+                        lkl_ctx_interface_sizes(G->ctx, &sizes[0], &sizes[1], &sizes[2]) == 0))) {
+            for (int k = 0; k < n; ++k) {
+                scores[k] = fold(G->family, out[2 * k], out[2 * k + 1], s[k]);
+                if (iface) scores[k] = restraint_fold(scores[k], &cnt[3 * (size_t)k], sizes);
+            }
             ++G->gpu_launches;
             G->scored += n;
             return 0;

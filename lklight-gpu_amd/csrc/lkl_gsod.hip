@@ -49,12 +49,27 @@ __device__ __forceinline__ double fold(int family, double e, double v, double s)
     return family == 3 ? -(ek + 0.1 * v) + s : -(ek + v);
 }
 
+// dna.rs:989-995 with scoring.rs:83-110 fractions from interface counts.
+__device__ __forceinline__ double restraint_fold(double score, const int* cnt, int3 sz) {
+#pragma clang fp contract(off)
+    const double perc_r = sz.x ? (double)cnt[0] / (double)sz.x : 0.0;
+    const double perc_l = sz.y ? (double)cnt[1] / (double)sz.y : 0.0;
+    const double inter = sz.z ? (double)cnt[2] / (double)sz.z : 0.0;
+    const double penalty = inter > 0.0 ? 999.0 * inter : 0.0;
+    score += perc_r * score + perc_l * score - penalty;
+    return score;
+}
+
 __global__ void luciferin_kernel(GsoView G, const double* __restrict__ out, const double* __restrict__ outS,
-                                 int family) {
+                                 const int* __restrict__ icnt, int3 isz, int family) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= G.n) return;
-    if (G.moved[i] || G.step[i] == 0u) G.score[i] = fold(family, out[2 * i], out[2 * i + 1], outS ? outS[i] : 0.0);
+    if (G.moved[i] || G.step[i] == 0u) {
+        double sc = fold(family, out[2 * i], out[2 * i + 1], outS ? outS[i] : 0.0);
+        if (icnt) sc = restraint_fold(sc, icnt + 3 * (size_t)i, isz);
+        G.score[i] = sc;
+    }
     G.luc[i] = (1.0 - kRho) * G.luc[i] + kGamma * G.score[i];
     G.step[i] += 1u;
 }
@@ -140,17 +155,17 @@ struct lkl_gsod_buf {
     double *pose = nullptr, *pose_nx = nullptr, *luc = nullptr, *vis = nullptr, *score = nullptr, *rnd = nullptr,
            *out = nullptr, *outS = nullptr;
     unsigned char* moved = nullptr;
-    int *nbc = nullptr, *swarm_of = nullptr, *swarm_off = nullptr;
+    int *nbc = nullptr, *swarm_of = nullptr, *swarm_off = nullptr, *icnt = nullptr;
     unsigned* step = nullptr;
     hipStream_t stream = nullptr;  // used when the caller passes none
     size_t c_pose = 0, c_pose_nx = 0, c_luc = 0, c_vis = 0, c_score = 0, c_rnd = 0, c_out = 0, c_outS = 0,
-           c_moved = 0, c_nbc = 0, c_swarm_of = 0, c_swarm_off = 0, c_step = 0;
+           c_moved = 0, c_nbc = 0, c_swarm_of = 0, c_swarm_off = 0, c_step = 0, c_icnt = 0;
 };
 
 void lkl_gsod_free(lkl_gsod_buf* d) {
     if (!d) return;
     void* p[] = {d->pose, d->pose_nx, d->luc, d->vis, d->score, d->rnd, d->out, d->outS,
-                 d->moved, d->nbc, d->swarm_of, d->swarm_off, d->step};
+                 d->moved, d->nbc, d->swarm_of, d->swarm_off, d->step, d->icnt};
     for (void* q : p)
         if (q) (void)hipFree(q);
     if (d->stream) (void)hipStreamDestroy(d->stream);
@@ -185,8 +200,11 @@ int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, in
     if ((e = grow(&d->field, &d->c_##field, (count))) != hipSuccess) return fail("alloc", e);
     GROW(pose, 7 * N) GROW(pose_nx, 7 * N) GROW(luc, N) GROW(vis, N) GROW(score, N) GROW(rnd, (size_t)steps * N)
     GROW(out, 2 * N) GROW(outS, N) GROW(moved, N) GROW(nbc, N) GROW(swarm_of, N) GROW(swarm_off, (size_t)n_swarms + 1)
-    GROW(step, N)
+    GROW(step, N) GROW(icnt, 3 * N)
 #undef GROW
+    const bool iface = lkl_ctx_has_interface(ctx) == 1;
+    int3 isz = make_int3(0, 0, 0);
+    if (iface && lkl_ctx_interface_sizes(ctx, &isz.x, &isz.y, &isz.z)) return fail("interface sizes", hipSuccess);
     struct Up { void* dst; const void* src; size_t bytes; };
     const Up up[] = {{d->pose, pose7, 56 * N}, {d->luc, luc, 8 * N}, {d->vis, vis, 8 * N}, {d->score, score, 8 * N},
                      {d->moved, moved, N}, {d->step, step, 4 * N}, {d->swarm_of, swarm_of.data(), 4 * N},
@@ -198,8 +216,9 @@ int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, in
     for (int k = 0; k < steps; ++k) {
         if (lkl_ctx_score_async(ctx, G.pose, n, flags, d->out, (flags & LKL_F_DESOLV) ? d->outS : nullptr, st))
             return fail("score", hipSuccess);
+        if (iface && lkl_ctx_interface_async(ctx, G.pose, n, d->icnt, st)) return fail("interface", hipSuccess);
         hipLaunchKernelGGL(luciferin_kernel, grid, block, 0, st, G, d->out,
-                           (flags & LKL_F_DESOLV) ? d->outS : nullptr, family);
+                           (flags & LKL_F_DESOLV) ? d->outS : nullptr, iface ? d->icnt : nullptr, isz, family);
         hipLaunchKernelGGL(move_kernel, grid, block, 0, st, G, d->rnd + (size_t)k * N);
         if ((e = hipGetLastError()) != hipSuccess) return fail("launch", e);
         std::swap(G.pose, G.pose_nx);

@@ -830,6 +830,91 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
     out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
 }
 
+// ── interface flags (restraints / membrane) ───────────────────────────────────
+// dna.rs:885-995 / cpydock.rs:509,639: a receptor and a ligand atom are at the
+// interface when their f64 squared distance is <= INTERFACE_CUTOFF^2
+// (constants.rs:14-15); satisfied_restraints / membrane_intersection
+// (scoring.rs:83-110) then count residues with an interface atom and membrane
+// beads at the interface.  One workgroup per pose: the ligand atoms walk their
+// neighbour lists (a superset of the 3.9 A pairs), pairs within 1e-3 A^2 of
+// the cutoff are re-measured in f64 as the CPU measures them, and the flags
+// are bits in LDS.  out[3k..3k+2] = satisfied receptor residues, satisfied
+// ligand residues, membrane beads at the interface (integers: exact).
+constexpr double IFACE2 = 3.9 * 3.9;
+constexpr float IFACE_HI = 15.211f, IFACE_LO = 15.209f;
+
+struct IfaceDev {
+    const int* r_off;
+    const int* r_atoms;  // sorted receptor indices
+    const int* l_off;
+    const int* l_atoms;  // sorted ligand indices
+    const int* mem;      // sorted receptor indices
+    int nres_r, nres_l, nmem;
+};
+
+__global__ void __launch_bounds__(512)
+iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, int* __restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) unsigned ibits[];
+    __shared__ int icnt[3];
+    const int pose = (int)blockIdx.x;
+    if (pose >= N) return;
+    const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
+    const int wr = (C.nr + 31) / 32, wl = (C.nl + 31) / 32;
+    unsigned* rb = ibits;
+    unsigned* lbits = ibits + wr;
+    for (int k = tid; k < wr + wl; k += bs) ibits[k] = 0u;
+    if (tid < 3) icnt[tid] = 0;
+    __syncthreads();
+    const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    for (int t = tid; t < C.nl; t += bs) {
+        const float4 lb = C.lxyzq[t];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        const int n = list_node(C, x, y, z);
+        if (n < 0) continue;
+        const unsigned b = C.noff[n], en = C.noff[n + 1];
+        const float xh = (float)x, yh = (float)y, zh = (float)z;
+        const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
+        bool hit = false;
+        for (unsigned p = b; p < en; ++p) {
+            const int id = (int)C.nidx[p];
+            const float4 r = C.rrec[2 * id];
+            const float dx = (xh - r.x) + xl, dy = (yh - r.y) + yl, dz = (zh - r.z) + zl;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= IFACE_HI && !(d2 > IFACE_LO && exact_d2(x, y, z, r) > IFACE2)) {
+                atomicOr(&rb[id >> 5], 1u << (id & 31));
+                hit = true;
+            }
+        }
+        if (hit) atomicOr(&lbits[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    for (int k = tid; k < I.nres_r; k += bs) {
+        for (int q = I.r_off[k]; q < I.r_off[k + 1]; ++q) {
+            const int a = I.r_atoms[q];
+            if ((rb[a >> 5] >> (a & 31)) & 1u) {
+                atomicAdd(&icnt[0], 1);
+                break;
+            }
+        }
+    }
+    for (int k = tid; k < I.nres_l; k += bs) {
+        for (int q = I.l_off[k]; q < I.l_off[k + 1]; ++q) {
+            const int a = I.l_atoms[q];
+            if ((lbits[a >> 5] >> (a & 31)) & 1u) {
+                atomicAdd(&icnt[1], 1);
+                break;
+            }
+        }
+    }
+    for (int k = tid; k < I.nmem; k += bs) {
+        const int a = I.mem[k];
+        if ((rb[a >> 5] >> (a & 31)) & 1u) atomicAdd(&icnt[2], 1);
+    }
+    __syncthreads();
+    if (tid < 3) out[3 * (size_t)pose + tid] = icnt[tid];
+}
+
 // Diagnostic: device transform of every (pose, ligand atom), sorted order.
 __global__ void transform_kernel(DevComplex C, const double* __restrict__ poses, int N,
                                  double* __restrict__ outxyz) {
@@ -885,6 +970,15 @@ struct lkl_ctx {
     unsigned char* d_lflag = nullptr;
     float *d_ldes = nullptr, *d_lasa = nullptr;
     std::vector<int> lperm;  // sorted position -> reference index
+    std::vector<int> rperm;  // receptor: sorted position -> reference index
+    // interface flags (restraints / membrane), reference atom order, CSR
+    bool has_iface = false;
+    std::vector<int> if_roff, if_ratoms, if_loff, if_latoms, if_mem;
+    int* d_iface = nullptr;      // translated to sorted indices: r_off, r_atoms, l_off, l_atoms, mem
+    size_t cap_iface = 0;
+    unsigned iface_version = ~0u;  // layout version the device copy was translated for
+    int* d_icounts = nullptr;
+    size_t cap_icounts = 0;
     int ncx = 1, ncy = 1, ncz = 1;
     double cox = 0, coy = 0, coz = 0;
     // per-call buffers (grown on demand, never shrunk)
@@ -1068,6 +1162,7 @@ static int build_receptor(lkl_ctx* c) {
     for (size_t k = 0; k < ncell; ++k) start[k + 1] += start[k];
     std::vector<int> cur(start.begin(), start.end() - 1), order(nr);
     for (int i = 0; i < nr; ++i) order[cur[cell[i]]++] = i;
+    c->rperm = order;
     // one extra record past the sorted atoms: the neighbour lists' padding
     // entry, far from every atom (excluded from desolvation by its flag)
     std::vector<float4> xyzq(nr + 1), par(nr + 1);
@@ -1265,7 +1360,7 @@ static void ctx_free(lkl_ctx* c) {
     void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
-                    c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx};
+                    c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1558,6 +1653,121 @@ LKL_API int lkl_ctx_set_work_sort(lkl_ctx* c, int on) {
         c->work_sort = on != 0;
         c->version++;  // batches built on this context refresh their views
     }
+    return 0;
+}
+
+// Interface data: receptor and ligand restraint residues as CSR over atom
+// indices (reference order) and the membrane beads (receptor atom indices).
+LKL_API int lkl_ctx_set_interface(lkl_ctx* c, const int* rec_res_off, const int* rec_res_atoms, int n_rec_res,
+                                  const int* lig_res_off, const int* lig_res_atoms, int n_lig_res,
+                                  const int* membrane_atoms, int n_membrane) {
+    if (!c) { set_err("lkl_ctx_set_interface: null context"); return -1; }
+    if (n_rec_res < 0 || n_lig_res < 0 || n_membrane < 0 || (n_rec_res && (!rec_res_off || !rec_res_atoms)) ||
+        (n_lig_res && (!lig_res_off || !lig_res_atoms)) || (n_membrane && !membrane_atoms)) {
+        set_err("lkl_ctx_set_interface: bad arguments");
+        return -1;
+    }
+    auto check_csr = [](const int* off, const int* atoms, int n, int bound) {
+        if (n == 0) return true;
+        if (off[0] != 0) return false;
+        for (int k = 0; k < n; ++k)
+            if (off[k + 1] < off[k]) return false;
+        for (int q = 0; q < off[n]; ++q)
+            if (atoms[q] < 0 || atoms[q] >= bound) return false;
+        return true;
+    };
+    if (!check_csr(rec_res_off, rec_res_atoms, n_rec_res, c->nr) ||
+        !check_csr(lig_res_off, lig_res_atoms, n_lig_res, c->nl)) {
+        set_err("lkl_ctx_set_interface: residue offsets must start at 0, not decrease, and index atoms in range");
+        return -1;
+    }
+    for (int k = 0; k < n_membrane; ++k)
+        if (membrane_atoms[k] < 0 || membrane_atoms[k] >= c->nr) {
+            set_err("lkl_ctx_set_interface: membrane atom %d out of range", membrane_atoms[k]);
+            return -1;
+        }
+    c->if_roff.assign(rec_res_off ? rec_res_off : nullptr, rec_res_off ? rec_res_off + n_rec_res + 1 : nullptr);
+    c->if_ratoms.assign(rec_res_atoms ? rec_res_atoms : nullptr, rec_res_atoms && n_rec_res ? rec_res_atoms + rec_res_off[n_rec_res] : rec_res_atoms);
+    c->if_loff.assign(lig_res_off ? lig_res_off : nullptr, lig_res_off ? lig_res_off + n_lig_res + 1 : nullptr);
+    c->if_latoms.assign(lig_res_atoms ? lig_res_atoms : nullptr, lig_res_atoms && n_lig_res ? lig_res_atoms + lig_res_off[n_lig_res] : lig_res_atoms);
+    c->if_mem.assign(membrane_atoms ? membrane_atoms : nullptr, membrane_atoms ? membrane_atoms + n_membrane : nullptr);
+    if (c->if_roff.empty()) c->if_roff.assign(1, 0);
+    if (c->if_loff.empty()) c->if_loff.assign(1, 0);
+    c->has_iface = n_rec_res > 0 || n_lig_res > 0 || n_membrane > 0;
+    c->iface_version = ~0u;
+    return 0;
+}
+
+// Device copy of the interface data in the current sorted layouts (rebuilt
+// after the receptor is re-sorted).  Returns the view, or -1.
+static int iface_view(lkl_ctx* c, IfaceDev* I) {
+    if (c->iface_version != c->version) {
+        std::vector<int> inv_r(c->nr), inv_l(c->nl);
+        for (int p = 0; p < c->nr; ++p) inv_r[c->rperm[p]] = p;
+        for (int s = 0; s < c->nl; ++s) inv_l[c->lperm[s]] = s;
+        std::vector<int> blob;
+        blob.insert(blob.end(), c->if_roff.begin(), c->if_roff.end());
+        for (int a : c->if_ratoms) blob.push_back(inv_r[a]);
+        blob.insert(blob.end(), c->if_loff.begin(), c->if_loff.end());
+        for (int a : c->if_latoms) blob.push_back(inv_l[a]);
+        for (int a : c->if_mem) blob.push_back(inv_r[a]);
+        if (dev_reserve(&c->d_iface, &c->cap_iface, blob.size())) return -1;
+        HIPCK("upload", h2d(c->d_iface, blob.data(), sizeof(int) * blob.size(), c->stream));
+        c->iface_version = c->version;
+    }
+    int* p = c->d_iface;
+    I->nres_r = (int)c->if_roff.size() - 1;
+    I->nres_l = (int)c->if_loff.size() - 1;
+    I->nmem = (int)c->if_mem.size();
+    I->r_off = p; p += c->if_roff.size();
+    I->r_atoms = p; p += c->if_ratoms.size();
+    I->l_off = p; p += c->if_loff.size();
+    I->l_atoms = p; p += c->if_latoms.size();
+    I->mem = p;
+    return 0;
+}
+
+// Per-pose interface counts (see iface_kernel) for N device poses into
+// d_counts (N x 3 ints) on `stream` (NULL = the context's stream).
+LKL_API int lkl_ctx_interface_async(lkl_ctx* c, const double* d_poses, int N, int* d_counts, void* stream) {
+    if (!c) { set_err("lkl_ctx_interface_async: null context"); return -1; }
+    if (N < 0 || (N > 0 && (!d_poses || !d_counts))) { set_err("lkl_ctx_interface_async: bad arguments"); return -1; }
+    if (!c->has_iface) { set_err("lkl_ctx_interface_async: no restraints or membrane set"); return -1; }
+    if (!c->list_entries) {
+        set_err("lkl_ctx_interface_async: interface flags need the neighbour lists (on, receptor <= 65,535 atoms)");
+        return -1;
+    }
+    if (N == 0) return 0;
+    IfaceDev I;
+    if (iface_view(c, &I)) return -1;
+    const size_t shm = sizeof(unsigned) * (size_t)((c->nr + 31) / 32 + (c->nl + 31) / 32);
+    if (shm + 1024 > c->lds_max) { set_err("lkl_ctx_interface_async: interface bitsets exceed LDS"); return -1; }
+    hipStream_t st = stream ? (hipStream_t)stream : c->stream;
+    hipLaunchKernelGGL(iface_kernel, dim3((unsigned)N), dim3(256), shm, st, dev_view(c), I, d_poses, N, d_counts);
+    HIPCK("launch", hipGetLastError());
+    return 0;
+}
+
+// Blocking host-buffer variant: counts (N x 3 ints).
+LKL_API int lkl_ctx_interface(lkl_ctx* c, const double* poses, int N, int* counts) {
+    if (!c || !poses || !counts || N <= 0) { set_err("lkl_ctx_interface: bad arguments"); return -1; }
+    HIPCK("probe", hipSetDevice(c->device));
+    if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
+    if (dev_reserve(&c->d_icounts, &c->cap_icounts, (size_t)N * 3)) return -1;
+    HIPCK("upload", h2d(c->d_poses, poses, sizeof(double) * 7 * (size_t)N, c->stream));
+    if (lkl_ctx_interface_async(c, c->d_poses, N, c->d_icounts, nullptr)) return -1;
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    HIPCK("readback", hipMemcpy(counts, c->d_icounts, sizeof(int) * 3 * (size_t)N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+LKL_API int lkl_ctx_has_interface(const lkl_ctx* c) { return c && c->has_iface ? 1 : 0; }
+
+LKL_API int lkl_ctx_interface_sizes(const lkl_ctx* c, int* n_rec_res, int* n_lig_res, int* n_membrane) {
+    if (!c) return -1;
+    if (n_rec_res) *n_rec_res = c->has_iface ? (int)c->if_roff.size() - 1 : 0;
+    if (n_lig_res) *n_lig_res = c->has_iface ? (int)c->if_loff.size() - 1 : 0;
+    if (n_membrane) *n_membrane = c->has_iface ? (int)c->if_mem.size() : 0;
     return 0;
 }
 

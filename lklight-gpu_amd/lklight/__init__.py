@@ -15,7 +15,7 @@ from ._capi import (F_CLASH, F_DESOLV, F_ELEC, F_FAR, FLAGS_DNA, FLAGS_PYDOCK, F
 FACTOR, EPSILON = 332.0, 4.0
 
 __all__ = ["Context", "Multi", "device_available", "lib", "last_error", "GpuError", "LibraryMissing",
-           "family_flags", "fold_scores", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
+           "family_flags", "fold_scores", "fold_restraints", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
            "FLAGS_DNA", "FLAGS_PYDOCK", "FLAGS_VDW"]
 
 
@@ -28,6 +28,31 @@ def family_flags(method: str) -> int:
     """gpu_family.rs:187-194: dna -> 7, pydock/cpydock -> 3, vdw -> 0, other -> 7."""
     return {"dna": FLAGS_DNA, "pydock": FLAGS_PYDOCK, "cpydock": FLAGS_PYDOCK,
             "vdw": FLAGS_VDW}.get(method, FLAGS_DNA)
+
+
+MEMBRANE_PENALTY = 999.0  # constants.rs:21
+
+
+def _csr(residues):
+    """List of atom-index lists -> (offsets, atoms) int32 arrays."""
+    off = np.zeros(len(residues) + 1, np.int32)
+    off[1:] = np.cumsum([len(r) for r in residues])
+    atoms = np.asarray([a for r in residues for a in r], np.int32)
+    return off, atoms
+
+
+def fold_restraints(scores, counts, n_rec_res, n_lig_res, n_membrane):
+    """dna.rs:989-995 on folded scores: score += perc_r*score + perc_l*score -
+    penalty, with perc = satisfied residues / residues (0 without residues)
+    and penalty = 999 * beads at the interface / beads when > 0
+    (scoring.rs:83-110).  counts: N x 3 from Context.interface()."""
+    s = np.asarray(scores, np.float64)
+    c = np.asarray(counts).reshape(-1, 3)
+    perc_r = c[:, 0] / n_rec_res if n_rec_res else np.zeros(len(s))
+    perc_l = c[:, 1] / n_lig_res if n_lig_res else np.zeros(len(s))
+    inter = c[:, 2] / n_membrane if n_membrane else np.zeros(len(s))
+    penalty = np.where(inter > 0.0, MEMBRANE_PENALTY * inter, 0.0)
+    return s + ((perc_r * s + perc_l * s) - penalty)
 
 
 def fold_scores(method: str, e_raw, v, s=None):
@@ -148,6 +173,28 @@ class Context:
     def set_list_spacing(self, spacing):
         """lkl_ctx_set_list_spacing: neighbour-list node spacing (A); 0 = cell-row walk."""
         _capi.check(lib().lkl_ctx_set_list_spacing(self.h, float(spacing)), "lkl_ctx_set_list_spacing")
+
+    def set_interface(self, rec_residues=(), lig_residues=(), membrane=()):
+        """lkl_ctx_set_interface: restraint residues (lists of atom indices, reference
+        order) of each side and the receptor membrane beads."""
+        ro, ra = _csr(list(rec_residues))
+        lo, la = _csr(list(lig_residues))
+        m = np.asarray(list(membrane), np.int32)
+        self._iface = (ro, ra, lo, la, m)
+        _capi.check(lib().lkl_ctx_set_interface(self.h, _capi.ptr(ro, C.c_int), _capi.ptr(ra, C.c_int),
+                                                len(ro) - 1, _capi.ptr(lo, C.c_int), _capi.ptr(la, C.c_int),
+                                                len(lo) - 1, _capi.ptr(m, C.c_int), len(m)),
+                    "lkl_ctx_set_interface")
+        self.iface_sizes = (len(ro) - 1, len(lo) - 1, len(m))
+
+    def interface(self, poses):
+        """lkl_ctx_interface -> counts (N x 3 int32): satisfied receptor residues,
+        satisfied ligand residues, membrane beads at the interface."""
+        p = _capi.as_f64(poses).reshape(-1, 7)
+        out = np.zeros((p.shape[0], 3), np.int32)
+        _capi.check(lib().lkl_ctx_interface(self.h, _capi.ptr(p, C.c_double), p.shape[0],
+                                            _capi.ptr(out, C.c_int)), "lkl_ctx_interface")
+        return out
 
     def set_phi_packed(self, on):
         """lkl_ctx_set_phi_packed: keep the corner-packed far field (8 corners per cell)."""

@@ -81,6 +81,12 @@ def lib():
         "lkl_ctx_set_work_sort": (C.c_int, [C.c_void_p, C.c_int]),
         "lkl_ctx_set_list_spacing": (C.c_int, [C.c_void_p, C.c_float]),
         "lkl_ctx_set_phi_packed": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_ctx_set_interface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_int, C.c_void_p, C.c_int]),
+        "lkl_ctx_has_interface": (C.c_int, [C.c_void_p]),
+        "lkl_ctx_interface_sizes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lkl_ctx_interface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
+        "lkl_ctx_interface_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
         "lkl_ctx_phi_packed": (C.c_int, [C.c_void_p]),
         "lkl_ctx_list_info": (C.c_longlong, [C.c_void_p, C.c_void_p]),
         "lkl_batch_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,

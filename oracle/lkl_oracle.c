@@ -503,6 +503,86 @@ void orc_dna_energy_grid(const orc_mol* rec, const orc_mol* lig, const orc_field
     crec_free(&cr);
 }
 
+/* dna.rs:885-995 (and cpydock.rs:509,639): the interface flags of one pose.
+ * The CPU near pass visits every pair of the 27 10-A cells around a ligand
+ * atom and flags both atoms when the f64 squared distance is at most
+ * INTERFACE_CUTOFF^2 (constants.rs:14-15); 3.9 A pairs always lie in that
+ * window, so this is the set of all pairs within 3.9 A. */
+void orc_interface_flags(const orc_mol* rec, const orc_mol* lig, const double* pose7,
+                         unsigned char* iface_r, unsigned char* iface_l) {
+    crec cr;
+    crec_build(&cr, rec->xyz, rec->n);
+    int nl = lig->n;
+    double* lc = (double*)malloc(sizeof(double) * 3 * (size_t)(nl > 0 ? nl : 1));
+    orc_transform(lig->xyz, nl, pose7, lc);
+    memset(iface_r, 0, (size_t)rec->n);
+    memset(iface_l, 0, (size_t)nl);
+    const double icut2 = 3.9 * 3.9;
+    double csp = 10.0, cox = cr.c_origin[0], coy = cr.c_origin[1], coz = cr.c_origin[2];
+    int64_t ncx = cr.nc[0], ncy = cr.nc[1], ncz = cr.nc[2];
+    for (int j = 0; j < nl; ++j) {
+        double x = lc[3 * j], y = lc[3 * j + 1], z = lc[3 * j + 2];
+        int64_t cxi = (int64_t)floor((x - cox) / csp), cyi = (int64_t)floor((y - coy) / csp),
+                czi = (int64_t)floor((z - coz) / csp);
+        cxi = cxi < 0 ? 0 : (cxi > ncx - 1 ? ncx - 1 : cxi);
+        cyi = cyi < 0 ? 0 : (cyi > ncy - 1 ? ncy - 1 : cyi);
+        czi = czi < 0 ? 0 : (czi > ncz - 1 ? ncz - 1 : czi);
+        for (int64_t dz = -1; dz <= 1; ++dz) {
+            int64_t iz = czi + dz;
+            if (iz < 0 || iz >= ncz) continue;
+            for (int64_t dy = -1; dy <= 1; ++dy) {
+                int64_t iy = cyi + dy;
+                if (iy < 0 || iy >= ncy) continue;
+                for (int64_t dx = -1; dx <= 1; ++dx) {
+                    int64_t ix = cxi + dx;
+                    if (ix < 0 || ix >= ncx) continue;
+                    int64_t cell = (iz * ncy + iy) * ncx + ix;
+                    for (int p = cr.cell_start[cell]; p < cr.cell_start[cell + 1]; ++p) {
+                        int i = cr.cell_atoms[p];
+                        const double* r = rec->xyz + 3 * i;
+                        double ddx = x - r[0], ddy = y - r[1], ddz = z - r[2];
+                        if (ddx * ddx + ddy * ddy + ddz * ddz <= icut2) {
+                            iface_r[i] = 1;
+                            iface_l[j] = 1;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    free(lc);
+    crec_free(&cr);
+}
+
+/* scoring.rs:83-98: fraction of residues with at least one interface atom
+ * (0 without residues); residue k = atoms[off[k] .. off[k+1]). */
+double orc_satisfied_restraints(const unsigned char* iface, const int* off, const int* atoms, int n_res) {
+    if (n_res <= 0) return 0.0;
+    int num = 0;
+    for (int k = 0; k < n_res; ++k)
+        for (int q = off[k]; q < off[k + 1]; ++q)
+            if (iface[atoms[q]] == 1) {
+                ++num;
+                break;
+            }
+    return (double)num / (double)n_res;
+}
+
+/* scoring.rs:100-110: fraction of membrane beads at the interface. */
+double orc_membrane_intersection(const unsigned char* iface, const int* atoms, int n) {
+    if (n <= 0) return 0.0;
+    int num = 0;
+    for (int k = 0; k < n; ++k) num += iface[atoms[k]];
+    return (double)num / (double)n;
+}
+
+/* dna.rs:989-995 (cpydock.rs:545-549): score += perc_r*score + perc_l*score - penalty. */
+double orc_restraint_fold(double score, double perc_r, double perc_l, double intersection) {
+    double penalty = intersection > 0.0 ? 999.0 * intersection : 0.0;
+    score += perc_r * score + perc_l * score - penalty;
+    return score;
+}
+
 /* vdw.rs / pydock.rs / cpydock.rs share the NearCells near loop. */
 static void nc_grid_with(int family, const nearcells* nc, const orc_mol* rec, const orc_mol* lig,
                          const orc_field* f, const double* pose7, orc_parts* out) {

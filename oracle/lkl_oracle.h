@@ -72,6 +72,13 @@ typedef struct {
 /* dna.rs:834-998 */
 void orc_dna_energy_grid(const orc_mol* rec, const orc_mol* lig, const orc_field* f,
                          const double* pose7, orc_parts* out);
+/* dna.rs:885-995 interface flags of one pose (pairs with f64 d2 <= 3.9^2). */
+void orc_interface_flags(const orc_mol* rec, const orc_mol* lig, const double* pose7,
+                         unsigned char* iface_r, unsigned char* iface_l);
+/* scoring.rs:83-98 / 100-110 and the fold of dna.rs:989-995. */
+double orc_satisfied_restraints(const unsigned char* iface, const int* off, const int* atoms, int n_res);
+double orc_membrane_intersection(const unsigned char* iface, const int* atoms, int n);
+double orc_restraint_fold(double score, double perc_r, double perc_l, double intersection);
 /* vdw.rs:73-151 */
 void orc_vdw_energy_grid(const orc_mol* rec, const orc_mol* lig,
                          const double* pose7, orc_parts* out);

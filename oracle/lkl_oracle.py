@@ -100,6 +100,16 @@ def lib():
                                            C.c_int]
         L.orc_nearcells_pair_count.restype = C.c_int64
         L.orc_nearcells_pair_count.argtypes = [_d, C.c_int, C.c_double]
+        L.orc_interface_flags.restype = None
+        L.orc_interface_flags.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d, C.POINTER(C.c_ubyte),
+                                          C.POINTER(C.c_ubyte)]
+        L.orc_satisfied_restraints.restype = C.c_double
+        L.orc_satisfied_restraints.argtypes = [C.POINTER(C.c_ubyte), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                               C.c_int]
+        L.orc_membrane_intersection.restype = C.c_double
+        L.orc_membrane_intersection.argtypes = [C.POINTER(C.c_ubyte), C.POINTER(C.c_int), C.c_int]
+        L.orc_restraint_fold.restype = C.c_double
+        L.orc_restraint_fold.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -206,6 +216,33 @@ class Oracle:
                           int(steps), int(nthreads), _p(pose), _p(luc), _p(sc), _p(vis),
                           _p(nnb, C.c_int))
         return {"pose": pose, "luciferin": luc, "scoring": sc, "vision": vis, "neighbors": nnb}
+
+    def interface(self, pose):
+        """dna.rs:885-995 interface flags of one pose -> (iface_r, iface_l) u8 arrays."""
+        p = np.ascontiguousarray(pose, np.float64)
+        ir = np.zeros(self.rec.arrs["xyz"].shape[0], np.uint8)
+        il = np.zeros(self.lig.arrs["xyz"].shape[0], np.uint8)
+        lib().orc_interface_flags(C.byref(self.rec.s), C.byref(self.lig.s), _p(p), _p(ir, C.c_ubyte),
+                                  _p(il, C.c_ubyte))
+        return ir, il
+
+    def restraint_score(self, score, pose, rec_residues, lig_residues, membrane):
+        """The folded score of one pose with restraints / membrane (dna.rs:985-995)."""
+        L = lib()
+        ir, il = self.interface(pose)
+
+        def csr(res):
+            off = np.zeros(len(res) + 1, np.int32)
+            off[1:] = np.cumsum([len(r) for r in res])
+            atoms = np.asarray([a for r in res for a in r] or [0], np.int32)
+            return off, atoms
+        ro, ra = csr(rec_residues)
+        lo, la = csr(lig_residues)
+        m = np.asarray(list(membrane) or [0], np.int32)
+        pr = L.orc_satisfied_restraints(_p(ir, C.c_ubyte), _p(ro, C.c_int), _p(ra, C.c_int), len(rec_residues))
+        pl = L.orc_satisfied_restraints(_p(il, C.c_ubyte), _p(lo, C.c_int), _p(la, C.c_int), len(lig_residues))
+        mi = L.orc_membrane_intersection(_p(ir, C.c_ubyte), _p(m, C.c_int), len(membrane))
+        return L.orc_restraint_fold(float(score), pr, pl, mi)
 
     def dna_exact(self, pose):
         p = np.ascontiguousarray(pose, np.float64)

@@ -229,6 +229,60 @@ def test_large_receptor_row_walk(gpu):
     ctx.close()
 
 
+def _iface_counts_oracle(orc, poses, rec_res, lig_res, membrane):
+    out = np.zeros((len(poses), 3), np.int64)
+    for k, pose in enumerate(poses):
+        ir, il = orc.interface(pose)
+        out[k, 0] = sum(int(ir[r].any()) for r in map(np.asarray, rec_res))
+        out[k, 1] = sum(int(il[r].any()) for r in map(np.asarray, lig_res))
+        out[k, 2] = int(ir[np.asarray(membrane)].sum())
+    return out
+
+
+def test_interface_counts_match_oracle(small):
+    """Restraint / membrane interface counts (pairs within 3.9 A, dna.rs:885-995)
+    equal the oracle's per pose as integers; they survive a receptor re-sort;
+    the folded restraint scores match the oracle's to 1e-5."""
+    cx, orc, poses = small
+    rng = np.random.default_rng(4)
+    starts = rng.choice(np.arange(0, cx.rec.n - 8, 8), 25, replace=False)
+    rec_res = [list(range(a, a + 8)) for a in starts]
+    lstarts = rng.choice(np.arange(0, cx.lig.n - 5, 5), 8, replace=False)
+    lig_res = [list(range(a, a + 5)) for a in lstarts]
+    membrane = sorted(rng.choice(cx.rec.n, 40, replace=False).tolist())
+    ctx = lklight.Context(cx)
+    assert not lib_has_iface(ctx)
+    ctx.set_interface(rec_res, lig_res, membrane)
+    assert lib_has_iface(ctx)
+    say("interface counts, 64 poses")
+    got = ctx.interface(poses)
+    want = _iface_counts_oracle(orc, poses, rec_res, lig_res, membrane)
+    assert np.array_equal(got, want), np.argwhere(got != want)[:5]
+    assert (want[:, 0] > 0).sum() >= 10 and want[:, 2].sum() > 0  # the poses do touch
+    ctx.set_tiles(3.4, 128, 0)  # re-sorted receptor: indices are re-translated
+    assert np.array_equal(ctx.interface(poses), want)
+    e, v, _ = ctx.score(poses, lklight.FLAGS_DNA)
+    sc = lklight.fold_restraints(lklight.fold_scores("dna", e, v), got, len(rec_res), len(lig_res),
+                                 len(membrane))
+    ref = orc.batch("dna", poses, nthreads=4)
+    ref_sc = np.array([orc.restraint_score(lklight.fold_scores("dna", ref["e"][k], ref["v"][k]), poses[k],
+                                           rec_res, lig_res, membrane) for k in range(len(poses))])
+    err = np.abs(sc - ref_sc) / np.maximum(np.abs(ref_sc), 1.0)
+    say("restraint-folded dna max rel err %.3e" % err.max())
+    assert err.max() <= 1e-5
+    with pytest.raises(lklight.GpuError, match="out of range"):
+        ctx.set_interface([], [], [cx.rec.n])
+    ctx.set_interface()  # cleared
+    assert not lib_has_iface(ctx)
+    with pytest.raises(lklight.GpuError, match="no restraints"):
+        ctx.interface(poses[:2])
+    ctx.close()
+
+
+def lib_has_iface(ctx):
+    return bool(lklight.lib().lkl_ctx_has_interface(ctx.h))
+
+
 def test_determinism_and_batch_split(small):
     cx, orc, poses = small
     say("determinism (multi-tile poses: accumulators must re-zero between launches)")

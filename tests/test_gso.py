@@ -234,3 +234,48 @@ def test_gso_device_run_matches_host_driver(gpu, method):
     for g in (host, dev, one, many):
         g.close()
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_gso_with_restraints_on_gpu(gpu):
+    """Restraints and a membrane keep the reference's GSO off the GPU batch
+    (dna.rs:1047-1056); here the GPU scores them (interface counts + the fold
+    of dna.rs:989-995).  The GPU-scored trajectory equals the one scored by the
+    oracle's restrained CPU path (a fallback-only driver) pose for pose over 8
+    steps, and the device-resident steps make the same moves."""
+    cx = synth.make_config("c1")
+    cx.build_field()
+    rng = np.random.default_rng(9)
+    rec_res = [list(range(a, a + 6)) for a in rng.choice(np.arange(0, cx.rec.n - 6, 6), 40, replace=False)]
+    lig_res = [list(range(a, a + 4)) for a in rng.choice(np.arange(0, cx.lig.n - 4, 4), 10, replace=False)]
+    membrane = sorted(rng.choice(cx.rec.n, 60, replace=False).tolist())
+    pos = synth.make_swarm_positions(2, 30, cx.meta["rec_axes"], 21, lig_radius=cx.meta["lig_axes"][0])
+    orc = O.Oracle(cx)
+
+    def cpu(poses):
+        r = orc.batch("dna", poses, nthreads=8)
+        return np.array([orc.restraint_score(lklight.fold_scores("dna", r["e"][k], r["v"][k]), poses[k],
+                                             rec_res, lig_res, membrane) for k in range(len(poses))])
+    ctx = lklight.Context(cx)
+    ctx.set_interface(rec_res, lig_res, membrane)
+    sizes = [30, 30]
+    gpu_g = G.GSO(ctx, "dna", pos, seed=5, swarm_sizes=sizes)
+    cpu_g = G.GSO(None, "dna", pos, seed=5, swarm_sizes=sizes, fallback=cpu)
+    dev_g = G.GSO(ctx, "dna", pos, seed=5, swarm_sizes=sizes)
+    plain = G.GSO(lklight.Context(cx), "dna", pos, seed=5, swarm_sizes=sizes)
+    for s in range(8):
+        for g in (gpu_g, cpu_g, plain):
+            g.update_luciferin()
+            g.movement_phase()
+        dev_g.device_run(1)
+        a, b, d = gpu_g.state(), cpu_g.state(), dev_g.state()
+        assert np.array_equal(a["pose"], b["pose"]), s
+        err = np.abs(a["scoring"] - b["scoring"]) / np.maximum(np.abs(b["scoring"]), 1.0)
+        assert err.max() <= 1e-5, (s, err.max())
+        assert np.array_equal(a["moved"], d["moved"]) and np.array_equal(a["neighbors"], d["neighbors"]), s
+        assert np.abs(a["pose"] - d["pose"]).max() <= 1e-9, s
+    # the restraints change the scores (and the search) against the plain run
+    assert not np.allclose(a["scoring"], plain.state()["scoring"])
+    for g in (gpu_g, cpu_g, dev_g, plain):
+        g.close()
+    ctx.close()
