@@ -836,7 +836,7 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
 // (constants.rs:14-15); satisfied_restraints / membrane_intersection
 // (scoring.rs:83-110) then count residues with an interface atom and membrane
 // beads at the interface.  One workgroup per pose: the ligand atoms walk their
-// neighbour lists (a superset of the 3.9 A pairs), pairs within 1e-3 A^2 of
+// nodes' short lists (a superset of the 3.9 A pairs), pairs within 1e-3 A^2 of
 // the cutoff are re-measured in f64 as the CPU measures them, and the flags
 // are bits in LDS.  out[3k..3k+2] = satisfied receptor residues, satisfied
 // ligand residues, membrane beads at the interface (integers: exact).
@@ -844,6 +844,8 @@ constexpr double IFACE2 = 3.9 * 3.9;
 constexpr float IFACE_HI = 15.211f, IFACE_LO = 15.209f;
 
 struct IfaceDev {
+    const unsigned* noff;          // short lists over the main lists' nodes
+    const unsigned short* nidx;
     const int* r_off;
     const int* r_atoms;  // sorted receptor indices
     const int* l_off;
@@ -872,12 +874,12 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
         transform(M, lb.x, lb.y, lb.z, x, y, z);
         const int n = list_node(C, x, y, z);
         if (n < 0) continue;
-        const unsigned b = C.noff[n], en = C.noff[n + 1];
+        const unsigned b = I.noff[n], en = I.noff[n + 1];
         const float xh = (float)x, yh = (float)y, zh = (float)z;
         const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
         bool hit = false;
         for (unsigned p = b; p < en; ++p) {
-            const int id = (int)C.nidx[p];
+            const int id = (int)I.nidx[p];
             const float4 r = C.rrec[2 * id];
             const float dx = (xh - r.x) + xl, dy = (yh - r.y) + yl, dz = (zh - r.z) + zl;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -979,6 +981,11 @@ struct lkl_ctx {
     unsigned iface_version = ~0u;  // layout version the device copy was translated for
     int* d_icounts = nullptr;
     size_t cap_icounts = 0;
+    // short neighbour lists for the interface pass (same nodes as the main
+    // lists, radius 3.9 A + half-diagonal + margin), built on first use
+    unsigned* d_noff_if = nullptr;
+    unsigned short* d_nidx_if = nullptr;
+    unsigned iflist_version = ~0u;
     int ncx = 1, ncy = 1, ncz = 1;
     double cox = 0, coy = 0, coz = 0;
     // per-call buffers (grown on demand, never shrunk)
@@ -1074,9 +1081,52 @@ static int build_phi8(lkl_ctx* c) {
 constexpr long long LIST_NODES_MAX = 1ll << 27;
 constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of indices
 
+// Count and fill one list set over the node grid (o, n, h) with radius rad;
+// outputs a fresh offsets array (nodes + 1) and index array.  Returns the
+// entry count, 0 when it exceeds LIST_ENTRIES_MAX (nothing allocated), or -1.
+static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float h, float rad, unsigned** d_off,
+                            unsigned short** d_idx) {
+    const long long nodes = (long long)n[0] * n[1] * n[2];
+    HIPCK("alloc", hipMalloc((void**)d_off, sizeof(unsigned) * (size_t)(nodes + 1)));
+    dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
+    hipLaunchKernelGGL(nlist_kernel<false>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
+                       n[0], n[1], n[2], rad, *d_off, (unsigned short*)nullptr);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    std::vector<unsigned> off((size_t)nodes + 1);
+    HIPCK("readback", hipMemcpy(off.data(), *d_off, sizeof(unsigned) * (size_t)nodes, hipMemcpyDeviceToHost));
+    unsigned long long acc = 0;
+    for (long long k = 0; k < nodes; ++k) {
+        const unsigned cnt = off[k];
+        off[k] = (unsigned)acc;
+        acc += cnt;
+        if (acc > (unsigned long long)LIST_ENTRIES_MAX) break;
+    }
+    if (acc > (unsigned long long)LIST_ENTRIES_MAX) {
+        (void)hipFree(*d_off);
+        *d_off = nullptr;
+        return 0;
+    }
+    off[nodes] = (unsigned)acc;
+    HIPCK("upload", h2d(*d_off, off.data(), sizeof(unsigned) * off.size(), c->stream));
+    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)std::max(acc, 4ull)));
+    hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
+                       n[0], n[1], n[2], rad, *d_off, *d_idx);
+    HIPCK("launch", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    return (long long)std::max(acc, 1ull);
+}
+
+static void free_iface_lists(lkl_ctx* c) {
+    if (c->d_noff_if) { (void)hipFree(c->d_noff_if); c->d_noff_if = nullptr; }
+    if (c->d_nidx_if) { (void)hipFree(c->d_nidx_if); c->d_nidx_if = nullptr; }
+    c->iflist_version = ~0u;
+}
+
 static int build_lists(lkl_ctx* c) {
     if (c->d_noff) { (void)hipFree(c->d_noff); c->d_noff = nullptr; }
     if (c->d_nidx) { (void)hipFree(c->d_nidx); c->d_nidx = nullptr; }
+    free_iface_lists(c);
     c->list_entries = 0;
     const int nr = c->nr;
     if (!(c->list_h > 0.0f) || nr <= 0 || nr > 65535) return 0;
@@ -1097,36 +1147,27 @@ static int build_lists(lkl_ctx* c) {
         nodes *= n[a];
     }
     if (nodes > LIST_NODES_MAX || (n[1] + NLY - 1) / NLY > 65535 || (n[2] + NLZ - 1) / NLZ > 65535) return 0;
-    HIPCK("alloc", hipMalloc((void**)&c->d_noff, sizeof(unsigned) * (size_t)(nodes + 1)));
-    dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
-    hipLaunchKernelGGL(nlist_kernel<false>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, c->d_noff, (unsigned short*)nullptr);
-    HIPCK("launch", hipGetLastError());
-    HIPCK("sync", hipStreamSynchronize(c->stream));
-    std::vector<unsigned> off((size_t)nodes + 1);
-    HIPCK("readback", hipMemcpy(off.data(), c->d_noff, sizeof(unsigned) * (size_t)nodes, hipMemcpyDeviceToHost));
-    unsigned long long acc = 0;
-    for (long long k = 0; k < nodes; ++k) {
-        const unsigned cnt = off[k];
-        off[k] = (unsigned)acc;
-        acc += cnt;
-        if (acc > (unsigned long long)LIST_ENTRIES_MAX) break;
-    }
-    if (acc > (unsigned long long)LIST_ENTRIES_MAX) {
-        (void)hipFree(c->d_noff);
-        c->d_noff = nullptr;
-        return 0;
-    }
-    off[nodes] = (unsigned)acc;
-    HIPCK("upload", h2d(c->d_noff, off.data(), sizeof(unsigned) * off.size(), c->stream));
-    HIPCK("alloc", hipMalloc((void**)&c->d_nidx, sizeof(unsigned short) * (size_t)std::max(acc, 4ull)));
-    hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, c->d_noff, c->d_nidx);
-    HIPCK("launch", hipGetLastError());
-    HIPCK("sync", hipStreamSynchronize(c->stream));
+    const long long entries = fill_lists(c, o, n, h, rad, &c->d_noff, &c->d_nidx);
+    if (entries <= 0) return (int)entries;
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
-    c->list_entries = (size_t)std::max(acc, 1ull);
+    c->list_entries = (size_t)entries;
     c->version++;
+    return 0;
+}
+
+// The interface pass's short lists, on the main lists' nodes (built on first
+// use after each receptor layout change).  Radius: every receptor atom whose
+// f32 squared distance can reach the pass's filter (15.211) from a point
+// within the node's half-diagonal, plus the same 0.01 A margin.
+static int ensure_iface_lists(lkl_ctx* c) {
+    if (c->iflist_version == c->version && c->d_noff_if) return 0;
+    free_iface_lists(c);
+    const float h = c->list_h;
+    const float rad = std::sqrt(15.211f) + 0.5f * std::sqrt(3.0f) * h + 0.01f;
+    const long long entries = fill_lists(c, c->no, c->nn, h, rad, &c->d_noff_if, &c->d_nidx_if);
+    if (entries < 0) return -1;
+    if (entries == 0) { set_err("interface lists exceed the entry cap"); return -1; }
+    c->iflist_version = c->version;
     return 0;
 }
 
@@ -1360,7 +1401,8 @@ static void ctx_free(lkl_ctx* c) {
     void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
-                    c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts};
+                    c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
+                    c->d_noff_if, c->d_nidx_if};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1739,7 +1781,9 @@ LKL_API int lkl_ctx_interface_async(lkl_ctx* c, const double* d_poses, int N, in
     }
     if (N == 0) return 0;
     IfaceDev I;
-    if (iface_view(c, &I)) return -1;
+    if (ensure_iface_lists(c) || iface_view(c, &I)) return -1;
+    I.noff = c->d_noff_if;
+    I.nidx = c->d_nidx_if;
     const size_t shm = sizeof(unsigned) * (size_t)((c->nr + 31) / 32 + (c->nl + 31) / 32);
     if (shm + 1024 > c->lds_max) { set_err("lkl_ctx_interface_async: interface bitsets exceed LDS"); return -1; }
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
