@@ -477,8 +477,9 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 // Near field over neighbour lists, lanes grouped for coalescing: pass 1
 // transforms every atom of [j0, j1), adds its far field and looks up its list
 // [b, e) (P1 atoms per thread per round, loads issued together); the atoms
-// with a non-empty list are then compacted in their (Morton) order, and the
-// wavefronts take groups of GA consecutive active atoms from a shared counter.
+// with a non-empty list are then compacted, bucketed by the number of steps
+// their list takes and in (Morton) order within a bucket, and the wavefronts
+// take groups of GA consecutive active atoms from a shared counter.
 // Within a wavefront each GL-lane subgroup walks one atom's list, lane r taking
 // entries r, r + GL, ..., r + (GK-1) GL of each GK*GL-entry step: the index
 // loads of a subgroup are contiguous, the records of neighbouring atoms'
@@ -490,6 +491,7 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 constexpr int GL = 4;        // lanes per list
 constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 constexpr int GK = 8;        // entries per lane per step
+constexpr int GB = 4;        // step-count buckets of the active atoms
 
 template <bool FAR, bool ELEC, bool CLASH, int DES>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
@@ -538,22 +540,57 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         }
     }
     __syncthreads();
-    // stable compaction of the active atoms (their order is kept)
+    // stable compaction of the active atoms into GB buckets by the number of
+    // GL*GK-entry steps their list takes (most steps first, 4 or more share
+    // one), each bucket in (Morton) atom order: a group's atoms then take the
+    // same number of steps and stay spatially close
     unsigned nact = 0;
-    for (int c0 = 0; c0 < nT; c0 += bs) {
-        const int t = c0 + tid;
-        const bool a = t < nT && ae[t] != ab[t];
-        const unsigned long long m = __ballot(a);
-        if (lane == 0) wtot[w] = (unsigned)__popcll(m);
+    {
+        __shared__ unsigned wb[GB][16], bbase[GB], brun[GB];
+        constexpr unsigned SE = (unsigned)(GL * GK);
+        if (tid < GB) bbase[tid] = 0u;
         __syncthreads();
-        unsigned off = nact, tot = 0;
-        for (int q = 0; q < nw; ++q) {
-            if (q < w) off += wtot[q];
-            tot += wtot[q];
+        for (int t = tid; t < nT; t += bs) {
+            const unsigned len = ae[t] - ab[t];
+            if (len) atomicAdd(&bbase[GB - (int)min((len + SE - 1u) / SE, (unsigned)GB)], 1u);
         }
-        if (a) act[off + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = (unsigned short)t;
-        nact += tot;
         __syncthreads();
+        if (tid == 0) {
+            unsigned acc0 = 0u;
+            for (int q = 0; q < GB; ++q) {
+                const unsigned c = bbase[q];
+                bbase[q] = acc0;
+                brun[q] = 0u;
+                acc0 += c;
+            }
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < nT; c0 += bs) {
+            const int t = c0 + tid;
+            const unsigned len = t < nT ? ae[t] - ab[t] : 0u;
+            const int bk = len ? GB - (int)min((len + SE - 1u) / SE, (unsigned)GB) : -1;
+            unsigned long long mine = 0ull;
+#pragma unroll
+            for (int q = 0; q < GB; ++q) {
+                const unsigned long long m = __ballot(bk == q);
+                if (bk == q) mine = m;
+                if (lane == 0) wb[q][w] = (unsigned)__popcll(m);
+            }
+            __syncthreads();
+            if (bk >= 0) {
+                unsigned off = bbase[bk] + brun[bk];
+                for (int q = 0; q < w; ++q) off += wb[bk][q];
+                act[off + (unsigned)__popcll(mine & ((1ull << lane) - 1ull))] = (unsigned short)t;
+            }
+            __syncthreads();
+            if (tid < GB) {
+                unsigned tot = 0u;
+                for (int q = 0; q < nw; ++q) tot += wb[tid][q];
+                brun[tid] += tot;
+            }
+            __syncthreads();
+        }
+        for (int q = 0; q < GB; ++q) nact += brun[q];
     }
     const unsigned ngroups = (nact + (unsigned)GA - 1u) / (unsigned)GA;
     const int sub = lane / GL, r = lane % GL;
