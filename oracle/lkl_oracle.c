@@ -13,6 +13,8 @@
  *   orc_crec_build          27-cell set == all pairs (invariant) + brute force
  *   orc_nearcells_*         nearcell.rs:119-159 invariant (xorshift seed 7, 300 pts)
  *   orc_transform           qt.rs KATs (rotation of (1,0,0)), identity exactness
+ *   orc_dna_energy_grid     1AZP KAT dna.rs:1149-1151 reproduced to its last digit
+ *   orc_pydock_energy_exact 1AZP KAT pydock.rs:578-582 (1e-8)
  *   orc_*_energy_grid       cell path == brute force (orc_bruteforce_parts) to 1e-12,
  *                           grid vs exact 30 A sum (dna.rs:1154-1179 tolerance 2 %)
  *                           vdw grid == brute-force exact (vdw.rs:351-378, 1e-9)
@@ -816,6 +818,36 @@ double orc_dna_energy_exact(const orc_mol* rec, const orc_mol* lig, const double
                     if (d < dmin) vp += CLASH_PENALTY_W * (dmin - d);
                 }
                 vi += vp;
+            }
+        }
+        te += ei;
+        tv += vi;
+    }
+    free(lc);
+    return -(te * FACTOR / EPSILON + tv);
+}
+
+/* pydock.rs:280-313 (restraint-free): receptor-major pair loop, every pair within
+ * 30 A contributes clamped electrostatics, pairs within 10 A the capped LJ; the
+ * per-receptor-atom partial sums are then added in receptor order. */
+double orc_pydock_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7) {
+    int nl = lig->n, nr = rec->n;
+    double* lc = (double*)malloc(sizeof(double) * 3 * (size_t)(nl > 0 ? nl : 1));
+    orc_transform(lig->xyz, nl, pose7, lc);
+    double te = 0.0, tv = 0.0;
+    for (int i = 0; i < nr; ++i) {
+        const double* r = rec->xyz + 3 * i;
+        double ei = 0.0, vi = 0.0;
+        for (int j = 0; j < nl; ++j) {
+            double dx = r[0] - lc[3 * j], dy = r[1] - lc[3 * j + 1], dz = r[2] - lc[3 * j + 2];
+            double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 <= ELEC_DIST_CUTOFF2)
+                ei += clampd(rec->ele[i] * lig->ele[j] / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+            if (d2 <= VDW_DIST_CUTOFF2) {
+                double vdw_e = rec->seps[i] * lig->seps[j];
+                double vdw_r = rec->rad[i] + lig->rad[j];
+                double p6 = powi6(vdw_r) / powi3(d2);
+                vi += fmin(vdw_e * (p6 * p6 - 2.0 * p6), VDW_CUTOFF);
             }
         }
         te += ei;

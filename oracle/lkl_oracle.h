@@ -107,6 +107,7 @@ void orc_bruteforce_parts(int family, const orc_mol* rec, const orc_mol* lig,
 
 /* dna.rs:668-819 energy_exact without restraints: 30 A all-pairs, no grid. */
 double orc_dna_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7);
+double orc_pydock_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7);
 
 /* nearcell.rs:22-112 self-test support: count pairs within cut2 found by the
  * 27-cell enumeration (NearCells with the given cell edge) over a point set. */

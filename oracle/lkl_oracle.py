@@ -76,6 +76,8 @@ def lib():
                                            C.POINTER(Field), _d, C.POINTER(Parts)]
         L.orc_dna_energy_exact.restype = C.c_double
         L.orc_dna_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
+        L.orc_pydock_energy_exact.restype = C.c_double
+        L.orc_pydock_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
         L.orc_rng_f64s.restype = None
         L.orc_rng_f64s.argtypes = [C.c_uint64, C.c_int, _d]
         L.orc_quat_random.restype = None
@@ -247,6 +249,10 @@ class Oracle:
     def dna_exact(self, pose):
         p = np.ascontiguousarray(pose, np.float64)
         return lib().orc_dna_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
+
+    def pydock_exact(self, pose):
+        p = np.ascontiguousarray(pose, np.float64)
+        return lib().orc_pydock_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
 
     def transform(self, pose):
         p = np.ascontiguousarray(pose, np.float64)

@@ -1,0 +1,152 @@
+"""The gfx950 HIP path on the 1AZP complex of the reference's own unit tests
+(a 1,094-atom protein receptor with hydrogens, a 506-atom DNA ligand; per-atom
+arrays from tests/golden/1azp.npz).
+
+The device is handed f32 coordinates and parameters, as the reference's GPU
+bridge hands them; the oracle runs on the same rounded values.  Bars: the
+device-built far field bitwise equal to the oracle's; per-pose scores within
+1e-5 relative (floor 1) of the oracle's CPU grid path; and the reference's own
+golden values (pydock.rs:578-592, dna.rs:1143-1178) met by the device's scores
+within the reference's tolerances.
+"""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+import lklight
+from lklight import _capi, synth
+from oracle import lkl_oracle as O
+
+import azp_fixture as A
+
+pytestmark = pytest.mark.gpu
+
+REL, FLOOR = 1e-5, 1.0
+
+
+def say(*a):
+    print("[gpu-test]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def poses_about_ligand(cx, n_small, n_any, seed):
+    """The reference's test poses plus rotations about the ligand centroid
+    (n_small within 20 degrees, n_any uniform) with N(0, 3 A) offsets, so
+    contacts range from the interpenetrated native pose to clear separation."""
+    rng = np.random.default_rng(seed)
+    c = cx.lig.xyz.mean(0)
+    q = synth.random_quaternions(n_small + n_any, rng)
+    ang = rng.uniform(0, np.radians(20), n_small)
+    axis = rng.normal(size=(n_small, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    q[:n_small] = np.concatenate([np.cos(ang / 2)[:, None], np.sin(ang / 2)[:, None] * axis], 1)
+    out = [A.pose7(t) for t in A.PYDOCK_POSES + A.DNA_POSES]
+    for k in range(q.shape[0]):
+        w, x, y, z = q[k]
+        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        t = c - R @ c + rng.normal(0, 3.0, 3)
+        out.append(np.concatenate([q[k], t]))
+    return np.asarray(out)
+
+
+@pytest.fixture(scope="module", params=["pydock", "dna"])
+def azp(request, gpu):
+    params = request.param
+    say("1azp", params, "parameters: f32 complex, device field")
+    cx = A.load(params, f32=True)
+    phi_o, o, d = O.field_build(cx.rec.xyz, cx.rec.ele)
+    phi_g = cx.build_field()
+    assert cx.origin == o and cx.dims == d
+    assert np.array_equal(phi_g.view(np.uint32), phi_o.view(np.uint32))
+    return params, cx, O.Oracle(cx), poses_about_ligand(cx, 120, 120, 11)
+
+
+def _check(method, ctx, orc, poses):
+    e, v, s = ctx.score(poses, lklight.family_flags(method))
+    ref = orc.batch(method, poses, nthreads=8)
+    g = lklight.fold_scores(method, e, v)
+    o = lklight.fold_scores(method, ref["e"], ref["v"])
+    err = np.abs(g - o) / np.maximum(np.abs(o), FLOOR)
+    say(method, "poses", poses.shape[0], "max rel err %.3e" % float(err.max()))
+    assert (err <= REL).all(), (method, float(err.max()), int(np.argmax(err)))
+    tol = REL * np.maximum(np.abs(o), FLOOR)
+    assert (np.abs(e - ref["e"]) * 83.0 <= tol).all()
+    assert (np.abs(v - ref["v"]) <= tol).all()
+    return g
+
+
+def test_1azp_parity(azp):
+    params, cx, orc, poses = azp
+    ctx = lklight.Context(cx)
+    methods = ("pydock", "vdw") if params == "pydock" else ("dna",)
+    for method in methods:
+        g = _check(method, ctx, orc, poses)
+        if method == "dna":
+            # dna.rs:1149-1151: the reference's grid energy at the identity pose, 2 %
+            g0 = g[len(A.PYDOCK_POSES)]
+            say("dna identity pose: device %.10f, reference %.10f" % (g0, A.DNA_ENERGY_IDENTITY))
+            assert abs(g0 - A.DNA_ENERGY_IDENTITY) <= 0.02 * abs(A.DNA_ENERGY_IDENTITY)
+            for k, t in enumerate(A.DNA_POSES):  # dna.rs:1168-1178, 2 % of exact
+                ex = orc.dna_exact(A.pose7(t))
+                assert abs(g[len(A.PYDOCK_POSES) + k] - ex) / max(abs(ex), 1.0) < 0.02, t
+        if method == "pydock":
+            for k, t in enumerate(A.PYDOCK_POSES):  # pydock.rs:583-592, 5 % of exact
+                ex = orc.pydock_exact(A.pose7(t))
+                r = abs(g[k] - ex) / abs(ex) if abs(ex) > 1.0 else abs(g[k] - ex)
+                assert r < 0.05, (t, g[k], ex)
+    ctx.close()
+
+
+def test_1azp_reference_call_surface(azp):
+    """lkl_batch_score with the reference's argument list (full_score.cu:359-371)
+    equals the context's scores bitwise; lkl_full_score (full_score.cu:140-150)
+    on the native pose's coordinates folds to the oracle's dna score within
+    1e-5 and to the reference's golden value within its 2 %."""
+    params, cx, orc, poses = azp
+    if params != "dna":
+        pytest.skip("one parameter set is enough for the wrappers")
+    L = lklight.lib()
+    keep = []
+
+    def fp(a, dt=np.float32, ct=C.c_float):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return _capi.ptr(a, ct)
+
+    def up(a):
+        return fp(a, np.uint8, C.c_ubyte)
+
+    dims, org = cx.dims, [float(v) for v in cx.origin]
+    cs, ca, nc, co = O.crec_build(cx.rec.xyz)
+    rec = [fp(cx.rec.xyz), fp(cx.rec.ele), fp(cx.rec.seps), fp(cx.rec.rad), up(cx.rec.heavy),
+           cx.rec.n, fp(cs, np.int32, C.c_int), fp(ca, np.int32, C.c_int), *nc, *co, 10.0]
+    P = np.ascontiguousarray(poses[:64], np.float64)
+    out = np.zeros((P.shape[0], 2))
+    flags = lklight.family_flags("dna")
+    say("lkl_batch_score on", P.shape[0], "poses")
+    rc = L.lkl_batch_score(fp(cx.phi), *dims, *org, 0.5, *rec, fp(cx.lig.xyz),
+                           fp(P, np.float64, C.c_double), fp(cx.lig.ele), fp(cx.lig.seps),
+                           fp(cx.lig.rad), up(cx.lig.heavy), cx.lig.n, P.shape[0], flags,
+                           fp(out, np.float64, C.c_double))
+    assert rc == 0, lklight.last_error()
+    ctx = lklight.Context(cx)
+    e, v, _ = ctx.score(P, flags)
+    ctx.close()
+    assert np.array_equal(out[:, 0], e) and np.array_equal(out[:, 1], v)
+    say("lkl_full_score on the native pose")
+    oe, ov = np.zeros(cx.lig.n, np.float32), np.zeros(cx.lig.n, np.float32)
+    rc = L.lkl_full_score(fp(cx.phi), *dims, *org, 0.5, *rec, fp(cx.lig.xyz), fp(cx.lig.ele),
+                          fp(cx.lig.seps), fp(cx.lig.rad), up(cx.lig.heavy), cx.lig.n,
+                          _capi.ptr(oe, C.c_float), _capi.ptr(ov, C.c_float))
+    assert rc == 0, lklight.last_error()
+    sc_g = -(oe.astype(np.float64).sum() * 83.0 + ov.astype(np.float64).sum())
+    ref = orc.parts("dna", A.pose7((0.0, 0.0, 0.0)))
+    sc_o = O.fold("dna", ref.e_raw, ref.v)
+    say("native pose: per-atom %.9f, oracle %.9f, reference %.9f" % (sc_g, sc_o, A.DNA_ENERGY_IDENTITY))
+    assert abs(sc_g - sc_o) <= REL * max(abs(sc_o), FLOOR)
+    assert abs(sc_g - A.DNA_ENERGY_IDENTITY) <= 0.02 * abs(A.DNA_ENERGY_IDENTITY)
+    L.lkl_cache_clear()  # the cache keys on host pointers that die with this test

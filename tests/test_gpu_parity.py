@@ -406,7 +406,7 @@ def test_edge_cases(small):
 
 
 def test_config1_parity(gpu):
-    """configs[0] complex: 1,500-atom receptor, 800-atom ligand, dna, 50 poses."""
+    """The 1,500-atom receptor / 800-atom ligand complex, all four families, 50 poses."""
     cx = synth.make_config("c1")
     say("c1 oracle field")
     phi, o, d = O.field_build(cx.rec.xyz, cx.rec.ele)

@@ -106,8 +106,8 @@ def test_gso_failure_without_fallback(small_vdw):
 
 @pytest.mark.gpu
 def test_gso_vdw_step_identical_gpu_vs_cpu(gpu):
-    """vdw GSO on the configs[0] complex (1,500-atom receptor, 800-atom ligand,
-    1 swarm x 50 glowworms x 20 steps): GPU-scored trajectory == CPU-grid-scored
+    """vdw GSO on the 1,500-atom receptor / 800-atom ligand complex
+    (1 swarm x 50 glowworms x 20 steps): GPU-scored trajectory == CPU-grid-scored
     trajectory at every step; per-pose scores within 1e-5 relative."""
     cx = synth.make_config("c1")
     pos = synth.make_swarm_positions(1, 50, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
@@ -196,7 +196,7 @@ def _rel(a, b):
 @pytest.mark.parametrize("method", ["dna", "vdw"])
 def test_gso_device_run_matches_host_driver(gpu, method):
     """Device-resident steps (lkl_gso_device_run) against the host driver from
-    the same start, 3 swarms x 40 glowworms on the configs[0] complex, 12 steps:
+    the same start, 3 swarms x 40 glowworms on the 1,500 x 800-atom complex, 12 steps:
     at every step the moves, neighbour counts and vision ranges are identical,
     poses agree to 1e-9 A (the device slerp's acos/sin are correctly rounded,
     the host libm's within one ulp) and scores / luciferin to 1e-7 relative.
