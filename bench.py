@@ -256,7 +256,8 @@ def main():
                                 % (args.family, N, "x".join(map(str, cx.dims))))
                    if args.workload == "c2" else
                    ("c3-cpydock: 8,000-atom receptor x 3,000-atom ligand (radii 1.5-2.0 A), %d poses/step, "
-                    "contact-SASA desolvation, phi %s at 0.5 A" % (N, "x".join(map(str, cx.dims)))),
+                    "contact-SASA desolvation over Lee & Richards areas (probe 1.4 A), phi %s at 0.5 A"
+                    % (N, "x".join(map(str, cx.dims)))),
                    "family": args.family, "flags": flags, "poses_per_step": N,
                    "cell_edge_A": tiles[0], "cell_edge_x_A": tiles[3], "block": tiles[1],
                    "ligand_atoms_per_tile": tiles[2],

@@ -273,6 +273,15 @@ long long lkl_field_dims(const float* r_xyz, int nr, double* origin3, int* dims3
  * bit.  Returns 0 / -1. */
 int lkl_field_build(const float* r_xyz, const float* r_ele, int nr, float* phi_out);
 
+/* Replaces lee_richards_sasa (lr_sasa.rs:277-291; cpydock.rs:287-332 calls it
+ * on each unbound monomer's heavy atoms), run on the GPU: out[i] = the Lee &
+ * Richards solvent-accessible area (A^2) of atom i, xyz n x 3 f64, radii
+ * without the probe.  The reference uses probe 1.4 and 20 slices.  f64 in the
+ * reference's operation order; neighbour order does not change an area (arcs
+ * are sorted before summing).  Host buffers, blocking.  Returns 0 / -1. */
+int lkl_lr_sasa(const double* xyz, const double* radii, int n, double probe, int n_slices,
+                double* out);
+
 /* Seeded random sequential addition of n atoms in the ellipsoid with semi-axes
  * (a, b, c) at minimum separation dmin, coordinates quantised to 2^-10 A
  * (synthetic benchmark inputs; no reference counterpart).  Returns the number

@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "lklight", "liblklight_gpu.so")
 
-SOURCES = ["lkl_score.hip", "lkl_field.hip", "lkl_surrogate.hip", "lkl_gsod.hip", "lkl_synth.cpp", "lkl_gso.cpp"]
+SOURCES = ["lkl_score.hip", "lkl_field.hip", "lkl_sasa.hip", "lkl_surrogate.hip", "lkl_gsod.hip", "lkl_synth.cpp", "lkl_gso.cpp"]
 
 
 def hipcc() -> str:

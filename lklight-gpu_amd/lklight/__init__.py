@@ -55,6 +55,21 @@ def fold_restraints(scores, counts, n_rec_res, n_lig_res, n_membrane):
     return s + ((perc_r * s + perc_l * s) - penalty)
 
 
+def lr_sasa(xyz, radii, probe=1.4, n_slices=20):
+    """Lee & Richards solvent-accessible area per atom (A^2) on the GPU
+    (lkl_lr_sasa; the reference's lr_sasa.rs:277-291 with its probe 1.4 and 20
+    slices by default).  xyz: (n, 3), radii without the probe."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    radii = np.ascontiguousarray(radii, np.float64)
+    if radii.shape != (xyz.shape[0],):
+        raise ValueError("radii must hold one value per atom")
+    out = np.zeros(xyz.shape[0])
+    _capi.check(lib().lkl_lr_sasa(_capi.ptr(xyz, C.c_double), _capi.ptr(radii, C.c_double),
+                                  xyz.shape[0], float(probe), int(n_slices),
+                                  _capi.ptr(out, C.c_double)), "lkl_lr_sasa")
+    return out
+
+
 def fold_scores(method: str, e_raw, v, s=None):
     """Per-pose scores from kernel parts: -(E*332/4 + V) (gpu_score.rs:326-331,
     gpu_family.rs:283-288); cpydock -(E*332/4 + 0.1 V) + S (cpydock.rs:789-791)."""

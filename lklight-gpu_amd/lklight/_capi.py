@@ -106,6 +106,7 @@ def lib():
                                     C.c_float, C.c_float, _f, _f, C.c_int, _d]),
         "lkl_field_dims": (C.c_longlong, [_f, C.c_int, _d, _i]),
         "lkl_field_build": (C.c_int, [_f, _f, C.c_int, _f]),
+        "lkl_lr_sasa": (C.c_int, [_d, _d, C.c_int, C.c_double, C.c_int, _d]),
         "lkl_synth_rsa": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_uint64, C.c_int64, _d, C.POINTER(C.c_int64)]),
         "lkl_ctx_build_field": (C.c_int, [C.c_void_p]),

@@ -142,7 +142,8 @@ def make_mol(n, axes, seed, net_charge, radii=None, center=False):
         rad = TYPE_RADIUS[t].astype(np.float64)
     else:
         rad = _f32(rng.uniform(radii[0], radii[1], n))
-    # exposure proxy for the reference SASA (non-negative), from the neighbour count
+    # exposure proxy for the reference SASA (non-negative), from the neighbour
+    # count; make_config replaces it with Lee & Richards areas for "c3"
     from scipy.spatial import cKDTree
     nb = np.array([len(v) - 1 for v in cKDTree(xyz).query_ball_point(xyz, 8.0)])
     area = 4.0 * math.pi * (rad + 1.4) ** 2 * np.clip(1.0 - nb / 30.0, 0.0, 1.0)
@@ -218,7 +219,15 @@ def make_config(name, seed=None, radii=None):
     rec, ra = make_mol(nr, raxes, seed, +10.0, radii=radii)
     lig, la = make_mol(nl, laxes, seed + 1, -10.0, radii=radii, center=True)
     meta = {"config": name, "seed": seed, "rec_axes": raxes, "lig_axes": laxes,
-            "rec_rsa_attempts": ra, "lig_rsa_attempts": la, "density": RHO}
+            "rec_rsa_attempts": ra, "lig_rsa_attempts": la, "density": RHO,
+            "asa": "neighbour-count proxy"}
+    if name == "c3":
+        # the desolvation workload's reference areas: Lee & Richards per
+        # monomer (probe 1.4 A, 20 slices, the atoms' radii), on the GPU
+        from . import lr_sasa
+        for m in (rec, lig):
+            m.asa = _f32(lr_sasa(m.xyz, m.rad))
+        meta["asa"] = "Lee & Richards, probe 1.4 A, 20 slices (lkl_lr_sasa)"
     return Complex(rec=rec, lig=lig, meta=meta)
 
 

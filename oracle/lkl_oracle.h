@@ -108,6 +108,12 @@ void orc_bruteforce_parts(int family, const orc_mol* rec, const orc_mol* lig,
 /* dna.rs:668-819 energy_exact without restraints: 30 A all-pairs, no grid. */
 double orc_dna_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7);
 double orc_pydock_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7);
+/* cpydock.rs:386-553 energy_exact without restraints (all pairs; desolvation). */
+double orc_cpydock_energy_exact(const orc_mol* rec, const orc_mol* lig, const double* pose7);
+
+/* lr_sasa.rs:277-291 Lee & Richards SASA per atom; radii without the probe. */
+void orc_lr_sasa(const double* xyz, const double* radii, int n, double probe, int n_slices,
+                 double* out);
 
 /* nearcell.rs:22-112 self-test support: count pairs within cut2 found by the
  * 27-cell enumeration (NearCells with the given cell edge) over a point set. */

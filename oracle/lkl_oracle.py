@@ -78,6 +78,10 @@ def lib():
         L.orc_dna_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
         L.orc_pydock_energy_exact.restype = C.c_double
         L.orc_pydock_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
+        L.orc_cpydock_energy_exact.restype = C.c_double
+        L.orc_cpydock_energy_exact.argtypes = [C.POINTER(Mol), C.POINTER(Mol), _d]
+        L.orc_lr_sasa.restype = None
+        L.orc_lr_sasa.argtypes = [_d, _d, C.c_int, C.c_double, C.c_int, _d]
         L.orc_rng_f64s.restype = None
         L.orc_rng_f64s.argtypes = [C.c_uint64, C.c_int, _d]
         L.orc_quat_random.restype = None
@@ -254,6 +258,10 @@ class Oracle:
         p = np.ascontiguousarray(pose, np.float64)
         return lib().orc_pydock_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
 
+    def cpydock_exact(self, pose):
+        p = np.ascontiguousarray(pose, np.float64)
+        return lib().orc_cpydock_energy_exact(C.byref(self.rec.s), C.byref(self.lig.s), _p(p))
+
     def transform(self, pose):
         p = np.ascontiguousarray(pose, np.float64)
         xyz = self.lig.arrs["xyz"]
@@ -289,6 +297,15 @@ def slerp(a, b, t):
 def nearcells_pair_count(xyz, cell):
     xyz = np.ascontiguousarray(xyz, np.float64)
     return lib().orc_nearcells_pair_count(_p(xyz), xyz.shape[0], float(cell))
+
+
+def lr_sasa(xyz, radii, probe=1.4, n_slices=20):
+    """lr_sasa.rs:277-291: Lee & Richards area per atom (radii without probe)."""
+    xyz = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    radii = np.ascontiguousarray(radii, np.float64)
+    out = np.zeros(xyz.shape[0])
+    lib().orc_lr_sasa(_p(xyz), _p(radii), xyz.shape[0], float(probe), int(n_slices), _p(out))
+    return out
 
 
 def crec_build(xyz):

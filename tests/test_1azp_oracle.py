@@ -89,6 +89,37 @@ def test_vdw_grid_equals_exact(pydock_cx, t):
     assert abs(g - e) < 1e-9, (t, g, e)
 
 
+@pytest.fixture(scope="module")
+def cpydock_cx():
+    cx = A.load("cpydock", sasa=O.lr_sasa)
+    cx.set_field(*O.field_build(cx.rec.xyz, cx.rec.ele))
+    return cx, O.Oracle(cx)
+
+
+def test_cpydock_areas(cpydock_cx):
+    """Every heavy atom has an area in [0, 4 pi (r + 1.4)^2] and the monomers
+    are not fully buried; hydrogens keep -1 (cpydock.rs:279-332)."""
+    cx, _ = cpydock_cx
+    z = np.load(A.PATH)
+    for m, k in ((cx.rec, "rec"), (cx.lig, "lig")):
+        heavy = m.hyd == 1
+        full = 4 * np.pi * (z[k + "_cpy_desrad"] + 1.4) ** 2
+        assert (m.asa[~heavy] == -1.0).all()
+        assert (m.asa[heavy] >= 0).all() and (m.asa[heavy] <= full[heavy] * (1 + 1e-12)).all()
+        assert (m.asa[heavy] > 0).mean() > 0.5
+
+
+@pytest.mark.parametrize("t", A.CPYDOCK_POSES)
+def test_cpydock_grid_vs_exact(cpydock_cx, t):
+    """cpydock.rs:861-875: grid and exact within 12 (absolute) at each pose."""
+    cx, orc = cpydock_cx
+    p = A.pose7(t)
+    parts = orc.parts("cpydock", p)
+    g = O.fold("cpydock", parts.e_raw, parts.v, parts.s)
+    e = orc.cpydock_exact(p)
+    assert abs(g - e) < A.CPYDOCK_GRID_EXACT, (t, g, e)
+
+
 def test_f32_inputs_stay_within_budget(dna_cx):
     """The device is handed f32 coordinates and parameters (the reference's
     GPU bridge casts them the same way); on this complex that rounding moves

@@ -101,6 +101,46 @@ def test_1azp_parity(azp):
     ctx.close()
 
 
+def test_1azp_cpydock(gpu):
+    """cpydock on 1AZP with the reference's areas (oracle Lee & Richards on the
+    f64 monomers): device scores against the oracle's grid path (bar below),
+    within the reference's 12 of the exact sum at cpydock.rs:861-875's poses;
+    the device's own areas (lkl_lr_sasa) agree with the oracle's."""
+    say("1azp cpydock: oracle areas, f32 complex, device field")
+    cx = A.load("cpydock", f32=True, sasa=O.lr_sasa)
+    dev = A.load("cpydock", f32=True, sasa=lklight.lr_sasa)
+    for m, d in ((cx.rec, dev.rec), (cx.lig, dev.lig)):
+        assert np.allclose(d.asa, m.asa, rtol=0, atol=1e-9)
+    cx.build_field()
+    orc = O.Oracle(cx)
+    poses = np.concatenate([np.asarray([A.pose7(t) for t in A.CPYDOCK_POSES]),
+                            poses_about_ligand(cx, 60, 60, 13)])
+    ctx = lklight.Context(cx, desolv=True)
+    flags = lklight.family_flags("cpydock") | lklight.F_DESOLV
+    e, v, s = ctx.score(poses, flags)
+    ctx.close()
+    ref = orc.batch("cpydock", poses, nthreads=8)
+    g = lklight.fold_scores("cpydock", e, v, s)
+    o = lklight.fold_scores("cpydock", ref["e"], ref["v"], ref["s"])
+    err = np.abs(g - o) / np.maximum(np.abs(o), FLOOR)
+    # Some poses of this complex score near 0 as the difference of terms of
+    # +-40..90 (E*83, 0.1 V, S).  The device's f32 pair arithmetic (the
+    # reference's GPU kernel computes in f32 as well) then leaves an error of
+    # ~2e-7 of those terms, which can exceed 1e-5 of a score below 1.  The bar
+    # is 1e-5 of max(|score|, 1) plus 1e-7 of the sum of the terms' magnitudes
+    # (the rounding bound of a sum evaluated in f32).
+    mag = np.abs(ref["e"]) * 83.0 + 0.1 * np.abs(ref["v"]) + np.abs(ref["s"])
+    tol = REL * np.maximum(np.abs(o), FLOOR) + 1e-7 * mag
+    say("cpydock poses", poses.shape[0], "max rel err %.3e, max err / bar %.3f"
+        % (float(err.max()), float((np.abs(g - o) / tol).max())))
+    assert (np.abs(g - o) <= tol).all(), int(np.argmax(np.abs(g - o) / tol))
+    well = np.abs(o) >= 0.05 * mag  # scores that are not a near-cancellation
+    assert (err[well] <= REL).all(), float(err[well].max())
+    for k, t in enumerate(A.CPYDOCK_POSES):
+        ex = orc.cpydock_exact(A.pose7(t))
+        assert abs(g[k] - ex) < A.CPYDOCK_GRID_EXACT, (t, g[k], ex)
+
+
 def test_1azp_reference_call_surface(azp):
     """lkl_batch_score with the reference's argument list (full_score.cu:359-371)
     equals the context's scores bitwise; lkl_full_score (full_score.cu:140-150)
