@@ -146,6 +146,20 @@ int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, doub
 int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
                         double* d_out, double* d_outS, void* stream);
 
+/* Scores ligands already placed: coords N*nl*3 f32 (pose-major, the
+ * descriptor's ligand order), one pose per coordinate set, the context's
+ * charges and parameters.  This is the input form of the reference's
+ * dump_metal_dataset poses.bin (dna.rs:473-663), which stores transformed
+ * coordinates rather than rotations; a rigidly placed set scores as the
+ * batched kernel scores the f32 coordinates of that placement.  Coordinates
+ * must be finite with |x| < 1e6.  Blocking; out / outS as lkl_ctx_score. */
+int lkl_ctx_score_coords(lkl_ctx* ctx, const float* coords, int N, unsigned flags, double* out,
+                         double* outS);
+
+/* The pose set of DNA::dump_metal_dataset (dna.rs:497-519) for a seed, from
+ * rand 0.7.3's StdRng: n x 7 (w, x, y, z, tx, ty, tz).  Returns 0 / -1. */
+int lkl_dataset_poses(unsigned long long seed, int n, double* out);
+
 /* Diagnostic: f64 transformed ligand coordinates of N poses (N*nl*3, ligand
  * order of the descriptor), computed by the device transform. */
 int lkl_ctx_transform(lkl_ctx* ctx, const double* poses, int N, double* out_xyz);

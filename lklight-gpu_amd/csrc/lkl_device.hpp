@@ -81,6 +81,9 @@ struct DevComplex {
     const float* ldes;
     const float* lasa;
     int nl;
+    // records per pose when each pose brings its own ligand coordinates
+    // (lxyzq then holds N x nl records, pose-major); 0 = one shared set
+    long long lstride;
     // far field (grid_dna.rs:33-40), x fastest
     const float* phi;
     int nx, ny, nz;

@@ -284,6 +284,34 @@ static int score_batch(lkl_gso* G, const std::vector<double>& poses, int n, std:
     return G->fallback(G->fallback_user, poses.data(), n, scores.data()) == 0 ? 0 : -1;
 }
 
+// The pose set of dna.rs:497-519 (DNA::dump_metal_dataset): per pose, three
+// gen::<f64>() for the translation (x 40 - 20) then quaternion candidates of
+// four gen::<f64>() (x 2 - 1, order w, x, y, z) until the squared norm exceeds
+// 1e-12, normalised by 1 / sqrt(norm2).  out: n x 7 (w, x, y, z, tx, ty, tz).
+LKL_API int lkl_dataset_poses(unsigned long long seed, int n, double* out) {
+#pragma clang fp contract(off)
+    if (n < 0 || (n > 0 && !out)) return -1;
+    StdRng rng(seed);
+    for (int k = 0; k < n; ++k) {
+        double t[3];
+        for (double& v : t) v = rng.next_f64() * 40.0 - 20.0;
+        double q[4];
+        for (;;) {
+            for (double& v : q) v = rng.next_f64() * 2.0 - 1.0;
+            const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+            if (n2 > 1e-12) {
+                const double inv = 1.0 / std::sqrt(n2);
+                for (double& v : q) v = v * inv;
+                break;
+            }
+        }
+        double* o = out + 7 * (size_t)k;
+        o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[3];
+        o[4] = t[0]; o[5] = t[1]; o[6] = t[2];
+    }
+    return 0;
+}
+
 LKL_API lkl_gso* lkl_gso_create(lkl_ctx* ctx, int family, int n_swarms, const int* swarm_sizes,
                                 const double* positions, unsigned long long seed,
                                 lkl_score_fallback fallback, void* fallback_user) {

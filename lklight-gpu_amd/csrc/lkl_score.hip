@@ -667,6 +667,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     const int j1 = min(C.nl, j0 + chunk);
     const int tid = threadIdx.x, bs = blockDim.x;
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    if (C.lstride) C.lxyzq += (size_t)pose * (size_t)C.lstride;
 
     unsigned* minR = nullptr;
     if (DES == DES_LDS) minR = lds_min;
@@ -1028,6 +1029,11 @@ struct lkl_ctx {
     // per-call buffers (grown on demand, never shrunk)
     double* d_poses = nullptr;
     size_t cap_poses = 0;
+    // per-pose ligand records of lkl_ctx_score_coords (N x nl, sorted order),
+    // and the ligand charges in sorted order that fill their w
+    float4* d_lcoords = nullptr;
+    size_t cap_lcoords = 0;
+    std::vector<float> lq;
     double* d_out = nullptr;
     size_t cap_out = 0;
     double* d_outS = nullptr;
@@ -1357,6 +1363,8 @@ static int build_ligand(lkl_ctx* c, const float* xyz, const float* ele, const fl
             la[s] = asa[j];
         }
     }
+    c->lq.resize(nl);
+    for (int s = 0; s < nl; ++s) c->lq[s] = lx[s].w;
     if (dev_upload(&c->d_lxyzq, lx.data(), (size_t)nl, c->stream)) return -1;
     if (dev_upload(&c->d_lpar, lp.data(), (size_t)nl, c->stream)) return -1;
     if (flag) {
@@ -1439,7 +1447,7 @@ static void ctx_free(lkl_ctx* c) {
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
-                    c->d_noff_if, c->d_nidx_if};
+                    c->d_noff_if, c->d_nidx_if, c->d_lcoords};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1551,10 +1559,14 @@ static size_t tile_lds(size_t nT, bool grouped) { return grouped ? 10 * nT + 16 
 
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
-                        double* d_outS, hipStream_t st, bool peratom) {
+                        double* d_outS, hipStream_t st, bool peratom, const float4* d_lcoords = nullptr) {
     if (N <= 0) return 0;
     if (validate_flags(c, flags)) return -1;
     DevComplex C = dev_view(c);
+    if (d_lcoords) {  // per-pose ligand records (lkl_ctx_score_coords)
+        C.lxyzq = d_lcoords;
+        C.lstride = c->nl;
+    }
     int des = DES_NONE;
     size_t shm = 0;
     // work-ordered near field: tile-local lists of (chunk x 3) bytes in LDS
@@ -1657,6 +1669,47 @@ LKL_API int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsi
     if (N < 0) { set_err("N must be >= 0"); return -1; }
     hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
     return launch_score(ctx, d_poses, N, flags, d_out, d_outS, st, false);
+}
+
+// The ligand already placed, per pose: coords N x nl x 3 f32 in the ligand's
+// input order.  Each pose gets its own records (sorted order, charge in w) and
+// the identity pose, whose f64 transform returns the f32 inputs exactly, so a
+// pose scores as the batched kernel scores those coordinates.  Blocking.
+LKL_API int lkl_ctx_score_coords(lkl_ctx* c, const float* coords, int N, unsigned flags, double* out,
+                                 double* outS) {
+    if (!c) { set_err("lkl_ctx_score_coords: null context"); return -1; }
+    if (N <= 0 || !coords || !out) { set_err("lkl_ctx_score_coords: bad arguments (N=%d)", N); return -1; }
+    if ((flags & F_DESOLV) && !outS) { set_err("F_DESOLV requires outS"); return -1; }
+    const size_t nl = (size_t)c->nl, n = (size_t)N * nl;
+    std::vector<float4> rec(n);
+    for (size_t k = 0; k < (size_t)N; ++k)
+        for (size_t s = 0; s < nl; ++s) {
+            const float* p = coords + (k * nl + (size_t)c->lperm[s]) * 3;
+            if (!(std::fabs(p[0]) < 1e6f && std::fabs(p[1]) < 1e6f && std::fabs(p[2]) < 1e6f)) {
+                set_err("lkl_ctx_score_coords: pose %zu atom %d has a non-finite or |x| >= 1e6 coordinate", k,
+                        c->lperm[s]);
+                return -1;
+            }
+            rec[k * nl + s] = make_float4(p[0], p[1], p[2], c->lq[s]);
+        }
+    std::vector<double> ident((size_t)N * 7, 0.0);
+    for (size_t k = 0; k < (size_t)N; ++k) ident[k * 7] = 1.0;
+    HIPCK("probe", hipSetDevice(c->device));
+    (void)hipGetLastError();
+    if (dev_reserve(&c->d_lcoords, &c->cap_lcoords, n)) return -1;
+    if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
+    if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
+    if ((flags & F_DESOLV) && dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
+    HIPCK("upload", h2d(c->d_lcoords, rec.data(), sizeof(float4) * n, c->stream));
+    HIPCK("upload", h2d(c->d_poses, ident.data(), sizeof(double) * ident.size(), c->stream));
+    if (launch_score(c, c->d_poses, N, flags, c->d_out, (flags & F_DESOLV) ? c->d_outS : nullptr, c->stream,
+                     false, c->d_lcoords))
+        return -1;
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    HIPCK("readback", hipMemcpy(out, c->d_out, sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToHost));
+    if (flags & F_DESOLV)
+        HIPCK("readback", hipMemcpy(outS, c->d_outS, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 LKL_API int lkl_ctx_build_field(lkl_ctx* c) {

@@ -154,6 +154,20 @@ class Context:
                     "lkl_ctx_score")
         return out[:, 0].copy(), out[:, 1].copy(), s
 
+    def score_coords(self, coords, flags):
+        """lkl_ctx_score_coords: ligands already placed, coords (N, nl, 3) in
+        the descriptor's ligand order -> (E_raw, V, S)."""
+        xyz = _capi.as_f32(coords)
+        n = xyz.size // (3 * self.nl) if self.nl else 0
+        if xyz.size != n * 3 * self.nl or n == 0:
+            raise ValueError("coords must hold N x nl x 3 values (nl = %d)" % self.nl)
+        out = np.zeros((n, 2), np.float64)
+        s = np.zeros(n, np.float64) if flags & F_DESOLV else None
+        _capi.check(lib().lkl_ctx_score_coords(self.h, _capi.ptr(xyz, C.c_float), n, flags,
+                                               _capi.ptr(out, C.c_double), _capi.ptr(s, C.c_double)),
+                    "lkl_ctx_score_coords")
+        return out[:, 0].copy(), out[:, 1].copy(), s
+
     def score_async(self, d_poses, n, flags, d_out, d_outS=None, stream=None):
         """Device pointers (ints), enqueued on `stream` (hipStream_t as int)."""
         _capi.check(lib().lkl_ctx_score_async(self.h, C.c_void_p(d_poses), n, flags,
