@@ -72,6 +72,9 @@ struct DevComplex {
     // the sorted receptor records interleaved for the list walk: rrec[2p] =
     // rxyzq[p], rrec[2p + 1] = rpar[p] (one 32-byte record per atom, nr + 1)
     const float4* rrec;
+    // position in nidx of LIST_PAD entries holding nr, past the last list: a
+    // lane whose list has ended loads its index from there
+    unsigned ndum;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
     // ligand, spatially sorted (Morton order of the reference coordinates)

@@ -112,18 +112,14 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
     if (d2 <= NEAR_HI && !(d2 > NEAR_LO && exact_d2(x64, y64, z64, r) > NEAR2)) {
         const float4 rp = PRE ? rp_pre : C.rpar[p];
         const float inv = __builtin_amdgcn_rcpf(d2);
-        if (ELEC) {  // clamp to +-ES_CAP (selects, no branches)
+        if (ELEC) {  // clamp to +-ES_CAP in fixed-point units (one integer median)
             const float ae = qjs * r.w * inv;
-            int te = __float2int_rn(ae);
-            te = ae > ES_CAP_SF ? (int)ES_CAP_I : te;
-            te = ae < -ES_CAP_SF ? -(int)ES_CAP_I : te;
-            e += te;
+            e += min(max(__float2int_rn(ae), -(int)ES_CAP_I), (int)ES_CAP_I);
         }
         const float svs = svjs * rp.x;
         const float rr = rj + rp.y;
-        const float rr2 = rr * rr;
-        const float i2 = inv * inv;
-        const float p6 = (rr2 * rr2 * rr2) * (i2 * inv);
+        const float t = (rr * rr) * inv;  // (r_i + r_j)^2 / d^2
+        const float p6 = t * t * t;
         float vp = svs * (p6 * p6 - 2.0f * p6);
         vp = fminf(vp, LJ_CAP * V_SCALE_F);
         if (CLASH && hj && rp.z != 0.0f) {
@@ -615,23 +611,33 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         const float xh = (float)x, yh = (float)y, zh = (float)z;
         const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
         double mloc = MIN_INIT;
-        for (unsigned p = b; __any(p < en); p += (unsigned)(GK * GL)) {
-            int id[GK];
+        // indices of the next step are requested before this step's pairs are
+        // formed, so a step waits for its records only
+        int id[GK];
 #pragma unroll
-            for (int k = 0; k < GK; ++k) {
-                const unsigned pos = p + (unsigned)(k * GL + r);
-                id[k] = pos < en ? (int)C.nidx[pos] : (int)dn;
-            }
+        for (int k = 0; k < GK; ++k) {
+            const unsigned pos = b + (unsigned)(k * GL + r);
+            id[k] = (int)C.nidx[pos < en ? pos : C.ndum];
+        }
+        for (unsigned p = b; __any(p < en); p += (unsigned)(GK * GL)) {
             float4 rq[GK], rp[GK];
 #pragma unroll
             for (int k = 0; k < GK; ++k) {
                 rq[k] = C.rrec[2 * id[k]];
                 rp[k] = C.rrec[2 * id[k] + 1];
             }
+            int idn[GK];
+#pragma unroll
+            for (int k = 0; k < GK; ++k) {
+                const unsigned pos = p + (unsigned)(GK * GL + k * GL + r);
+                idn[k] = (int)C.nidx[pos < en ? pos : C.ndum];
+            }
 #pragma unroll
             for (int k = 0; k < GK; ++k)
                 pair_term<ELEC, CLASH, DES, true>(C, rq[k], id[k], x, y, z, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
                                                   hj, lf, e, v, mloc, minR, rp[k]);
+#pragma unroll
+            for (int k = 0; k < GK; ++k) id[k] = idn[k];
         }
         if (DES != DES_NONE) {
 #pragma unroll
@@ -991,6 +997,7 @@ struct lkl_ctx {
     int nn[3] = {0, 0, 0};
     float no[3] = {0.f, 0.f, 0.f};
     size_t list_entries = 0;  // 0 = no lists (row walk)
+    unsigned list_dummy = 0;  // position of the LIST_PAD dummy entries in d_nidx
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -1128,7 +1135,7 @@ constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of indices
 // outputs a fresh offsets array (nodes + 1) and index array.  Returns the
 // entry count, 0 when it exceeds LIST_ENTRIES_MAX (nothing allocated), or -1.
 static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float h, float rad, unsigned** d_off,
-                            unsigned short** d_idx) {
+                            unsigned short** d_idx, unsigned* dummy_pos = nullptr) {
     const long long nodes = (long long)n[0] * n[1] * n[2];
     HIPCK("alloc", hipMalloc((void**)d_off, sizeof(unsigned) * (size_t)(nodes + 1)));
     dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
@@ -1152,7 +1159,12 @@ static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float 
     }
     off[nodes] = (unsigned)acc;
     HIPCK("upload", h2d(*d_off, off.data(), sizeof(unsigned) * off.size(), c->stream));
-    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)std::max(acc, 4ull)));
+    // LIST_PAD dummy entries (index nr) after the last list
+    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)(acc + LIST_PAD)));
+    const unsigned short dum[LIST_PAD] = {(unsigned short)c->nr, (unsigned short)c->nr, (unsigned short)c->nr,
+                                          (unsigned short)c->nr};
+    HIPCK("upload", h2d(*d_idx + acc, dum, sizeof dum, c->stream));
+    if (dummy_pos) *dummy_pos = (unsigned)acc;
     hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
                        n[0], n[1], n[2], rad, *d_off, *d_idx);
     HIPCK("launch", hipGetLastError());
@@ -1190,7 +1202,7 @@ static int build_lists(lkl_ctx* c) {
         nodes *= n[a];
     }
     if (nodes > LIST_NODES_MAX || (n[1] + NLY - 1) / NLY > 65535 || (n[2] + NLZ - 1) / NLZ > 65535) return 0;
-    const long long entries = fill_lists(c, o, n, h, rad, &c->d_noff, &c->d_nidx);
+    const long long entries = fill_lists(c, o, n, h, rad, &c->d_noff, &c->d_nidx, &c->list_dummy);
     if (entries <= 0) return (int)entries;
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
     c->list_entries = (size_t)entries;
@@ -1409,6 +1421,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
     if (c->list_entries) {
         C.noff = c->d_noff;
         C.nidx = c->d_nidx;
+        C.ndum = c->list_dummy;
         C.nnx = c->nn[0];
         C.nny = c->nn[1];
         C.nnz = c->nn[2];
