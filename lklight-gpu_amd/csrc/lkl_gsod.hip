@@ -13,6 +13,7 @@
 // slerp's acos and sin are correctly rounded (lkl_gsomath.hpp).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <vector>
 
@@ -74,9 +75,41 @@ __global__ void luciferin_kernel(GsoView G, const double* __restrict__ out, cons
     G.step[i] += 1u;
 }
 
-// One glowworm per thread.  Reads the step's snapshot (pose, luc, vis of all
-// glowworms of the swarm) and writes only its own next pose, moved flag,
-// neighbour count and vision.
+// glowworm.rs:128-150 and the vision update: glowworm i moves towards nid
+// (nid == i: stays) having seen cnt neighbours.  Reads the step's snapshot and
+// writes only glowworm i's next pose, moved flag, neighbour count and vision.
+__device__ void move_to(const GsoView& G, int i, int nid, int cnt) {
+#pragma clang fp contract(off)
+    const double* pi = G.pose + 7 * (size_t)i;
+    const double vi = G.vis[i];
+    const double tx = pi[4], ty = pi[5], tz = pi[6];
+    double* po = G.pose_nx + 7 * (size_t)i;
+    if (nid != i) {
+        const double* pn = G.pose + 7 * (size_t)nid;
+        const double dx = pn[4] - tx, dy = pn[5] - ty, dz = pn[6] - tz;
+        const double norm = sqrt(dx * dx + dy * dy + dz * dz);
+        const double coef = kTransStep / norm;
+        double t0 = tx, t1 = ty, t2 = tz;
+        t0 += dx * coef;
+        t1 += dy * coef;
+        t2 += dz * coef;
+        double q[4];
+        lkl::gm::slerp_cr(pi, pn, kRotStep, q);
+        lkl::gm::normalize4(q[0], q[1], q[2], q[3]);
+        po[0] = q[0]; po[1] = q[1]; po[2] = q[2]; po[3] = q[3];
+        po[4] = t0; po[5] = t1; po[6] = t2;
+    } else {
+        for (int c = 0; c < 7; ++c) po[c] = pi[c];
+    }
+    G.moved[i] = nid != i ? 1 : 0;
+    G.nbc[i] = cnt;
+    G.vis[i] = fmin(kMaxVision, fmax(0.0, vi + kBeta * (double)(kMaxNeighbors - cnt)));
+}
+
+// glowworm.rs:84-126, one lane per glowworm (swarms larger than the LDS list
+// of move_wave_kernel): the brighter members within the vision range in id
+// order, their total, and the member chosen by walking the cumulative
+// probabilities with the step's draw.
 __global__ void move_kernel(GsoView G, const double* __restrict__ rnd) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -98,7 +131,7 @@ __global__ void move_kernel(GsoView G, const double* __restrict__ rnd) {
         }
     }
     int nid = i;
-    if (cnt > 0) {  // glowworm.rs:114-126: walk the cumulative probabilities
+    if (cnt > 0) {
         const double r = rnd[i];
         double sum = 0.0;
         int last = -1;
@@ -114,27 +147,63 @@ __global__ void move_kernel(GsoView G, const double* __restrict__ rnd) {
         }
         nid = last >= 0 ? last : first;
     }
-    double* po = G.pose_nx + 7 * (size_t)i;
-    if (nid != i) {  // glowworm.rs:128-150
-        const double* pn = G.pose + 7 * (size_t)nid;
-        const double dx = pn[4] - tx, dy = pn[5] - ty, dz = pn[6] - tz;
-        const double norm = sqrt(dx * dx + dy * dy + dz * dz);
-        const double coef = kTransStep / norm;
-        double t0 = tx, t1 = ty, t2 = tz;
-        t0 += dx * coef;
-        t1 += dy * coef;
-        t2 += dz * coef;
-        double q[4];
-        lkl::gm::slerp_cr(pi, pn, kRotStep, q);
-        lkl::gm::normalize4(q[0], q[1], q[2], q[3]);
-        po[0] = q[0]; po[1] = q[1]; po[2] = q[2]; po[3] = q[3];
-        po[4] = t0; po[5] = t1; po[6] = t2;
-    } else {
-        for (int c = 0; c < 7; ++c) po[c] = pi[c];
+    move_to(G, i, nid, cnt);
+}
+
+// The same selection with one wavefront per glowworm: the 64 lanes test 64
+// members of the swarm at a time and those that pass are appended to an LDS
+// list in id order (ballot + prefix count).  Lane 0 then forms the total and
+// walks the cumulative probabilities over that list in id order -- the sums
+// and their order are move_kernel's -- and moves the glowworm.  A lane per
+// glowworm would wait on a dependent load per member; here the loads of a
+// whole chunk are in flight together.  Swarms of up to kSwarmList members.
+constexpr int kSwarmList = 1024;
+__global__ void __launch_bounds__(64) move_wave_kernel(GsoView G, const double* __restrict__ rnd) {
+#pragma clang fp contract(off)
+    __shared__ int nb_id[kSwarmList];
+    __shared__ double nb_l[kSwarmList];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    const int s = G.swarm_of[i], b = G.swarm_off[s], e = G.swarm_off[s + 1];
+    const double* pi = G.pose + 7 * (size_t)i;
+    const double li = G.luc[i], vr2 = G.vis[i] * G.vis[i];
+    const double tx = pi[4], ty = pi[5], tz = pi[6];
+    int cnt = 0;
+    for (int k0 = b; k0 < e; k0 += 64) {
+        const int k = k0 + lane;
+        bool pass = false;
+        double lk = 0.0;
+        if (k < e && k != i) {
+            lk = G.luc[k];
+            const double* pk = G.pose + 7 * (size_t)k;
+            const double dx = tx - pk[4], dy = ty - pk[5], dz = tz - pk[6];
+            pass = li < lk && dx * dx + dy * dy + dz * dz < vr2;
+        }
+        const unsigned long long m = __ballot(pass);
+        if (pass) {  // at < cnt + popcount(m) <= e - b - 1 < kSwarmList
+            const int at = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            nb_id[at] = k;
+            nb_l[at] = lk;
+        }
+        cnt += __popcll(m);
     }
-    G.moved[i] = nid != i ? 1 : 0;
-    G.nbc[i] = cnt;
-    G.vis[i] = fmin(kMaxVision, fmax(0.0, vi + kBeta * (double)(kMaxNeighbors - cnt)));
+    __syncthreads();
+    if (lane != 0) return;
+    int nid = i;
+    if (cnt > 0) {
+        double total = 0.0;
+        for (int q = 0; q < cnt; ++q) total += nb_l[q] - li;
+        const double r = rnd[i];
+        double sum = 0.0;
+        int last = -1;
+        for (int q = 0; q < cnt && sum < r; ++q) {
+            double p = nb_l[q] - li;
+            p /= total;
+            sum += p;
+            last = nb_id[q];
+        }
+        nid = last >= 0 ? last : nb_id[0];
+    }
+    move_to(G, i, nid, cnt);
 }
 
 template <typename T>
@@ -213,13 +282,17 @@ int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, in
         if ((e = hipMemcpyAsync(u.dst, u.src, u.bytes, hipMemcpyHostToDevice, st)) != hipSuccess) return fail("upload", e);
     GsoView G{d->pose, d->pose_nx, d->luc, d->vis, d->score, d->moved, d->nbc, d->step, d->swarm_of, d->swarm_off, n};
     const dim3 grid((unsigned)((N + 127) / 128)), block(128);
+    int smax = 0;
+    for (int q = 0; q < n_swarms; ++q) smax = std::max(smax, swarm_off[q + 1] - swarm_off[q]);
+    const bool listed = smax <= kSwarmList;  // every swarm fits the LDS list
     for (int k = 0; k < steps; ++k) {
         if (lkl_ctx_score_async(ctx, G.pose, n, flags, d->out, (flags & LKL_F_DESOLV) ? d->outS : nullptr, st))
             return fail("score", hipSuccess);
         if (iface && lkl_ctx_interface_async(ctx, G.pose, n, d->icnt, st)) return fail("interface", hipSuccess);
         hipLaunchKernelGGL(luciferin_kernel, grid, block, 0, st, G, d->out,
                            (flags & LKL_F_DESOLV) ? d->outS : nullptr, iface ? d->icnt : nullptr, isz, family);
-        hipLaunchKernelGGL(move_kernel, grid, block, 0, st, G, d->rnd + (size_t)k * N);
+        if (listed) hipLaunchKernelGGL(move_wave_kernel, dim3((unsigned)N), dim3(64), 0, st, G, d->rnd + (size_t)k * N);
+        else hipLaunchKernelGGL(move_kernel, grid, block, 0, st, G, d->rnd + (size_t)k * N);
         if ((e = hipGetLastError()) != hipSuccess) return fail("launch", e);
         std::swap(G.pose, G.pose_nx);
     }
