@@ -142,9 +142,19 @@ int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, doub
 
 /* Asynchronous, device buffers, enqueued on `stream` (a hipStream_t; NULL =
  * the context's own stream).  d_out / d_outS as above; d_outS may be NULL
- * without LKL_F_DESOLV.  Launches exactly one kernel. */
+ * without LKL_F_DESOLV.  Launches the scoring kernel, preceded, when the
+ * launch is dispatched longest first (lkl_ctx_set_pose_order), by two short
+ * kernels that estimate each pose's cost and order the poses; no copies or
+ * memsets, so the call can be captured in a graph. */
 int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
                         double* d_out, double* d_outS, void* stream);
+
+/* Longest-first dispatch (default on): one-tile-per-pose launches of 513 to
+ * 16,384 poses over neighbour lists first estimate each pose's near-field
+ * cost (list lengths at its atoms' nodes) and start the costly poses first,
+ * so cheap poses fill the last round of workgroup slots.  Scores are the same
+ * bits either way.  0 turns it off (one kernel per launch).  0 / -1. */
+int lkl_ctx_set_pose_order(lkl_ctx* ctx, int enable);
 
 /* Scores ligands already placed: coords N*nl*3 f32 (pose-major, the
  * descriptor's ligand order), one pose per coordinate set, the context's

@@ -87,6 +87,9 @@ struct DevComplex {
     // records per pose when each pose brings its own ligand coordinates
     // (lxyzq then holds N x nl records, pose-major); 0 = one shared set
     long long lstride;
+    // dispatch order of the poses (NULL = pose = tile): tile L scores pose
+    // porder[L]; set for one-tile-per-pose launches ordered by expected cost
+    const int* porder;
     // far field (grid_dna.rs:33-40), x fastest
     const float* phi;
     int nx, ny, nz;

@@ -85,6 +85,8 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 #define SQRT_D(x) __builtin_amdgcn_sqrtf(x)
 #endif
 constexpr int NU = LKL_NU;
+// one-tile-per-pose launches above this many poses are dispatched longest first
+constexpr int POSE_ORDER_MIN = 512;
 
 // One receptor candidate r (sorted index p) against ligand atom (x64, y64,
 // z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
@@ -668,7 +670,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     const unsigned total = (unsigned)N * (unsigned)nchunks;
     if (blockIdx.x >= total) return;
     const unsigned L = xcd_tile(blockIdx.x, total);
-    const int pose = (int)(L / (unsigned)nchunks);
+    const int pose = C.porder ? C.porder[L] : (int)(L / (unsigned)nchunks);
     const int j0 = (int)(L % (unsigned)nchunks) * chunk;
     const int j1 = min(C.nl, j0 + chunk);
     const int tid = threadIdx.x, bs = blockDim.x;
@@ -961,6 +963,62 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
     if (tid < 3) out[3 * (size_t)pose + tid] = icnt[tid];
 }
 
+// Expected near-field cost of each pose, for the dispatch order: one wavefront
+// per pose sums the neighbour-list lengths at the nodes of every fourth ligand
+// atom (the walk's work is proportional to the candidates it visits).
+constexpr int COST_STRIDE = 4;
+__global__ void __launch_bounds__(64) pose_cost_kernel(DevComplex C, const double* __restrict__ poses, int N,
+                                                       unsigned* __restrict__ cost) {
+    const int pose = blockIdx.x;
+    if (pose >= N) return;
+    const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    const float4* lx = C.lxyzq + (C.lstride ? (size_t)pose * (size_t)C.lstride : 0);
+    unsigned c = 0;
+    for (int j = (int)threadIdx.x * COST_STRIDE; j < C.nl; j += 64 * COST_STRIDE) {
+        const float4 lb = lx[j];
+        double x, y, z;
+        transform(M, lb.x, lb.y, lb.z, x, y, z);
+        const int n = list_node(C, x, y, z);
+        if (n >= 0) c += C.noff[n + 1] - C.noff[n];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if (threadIdx.x == 0) cost[pose] = c;
+}
+
+// Longest-first order within each XCD's range of tiles (xcd_tile gives XCD x
+// the contiguous range it dispatches in ascending order): block x sorts its
+// range by cost, descending, ties by pose, with a bitonic sort in LDS, so the
+// costly poses start first and the cheap ones fill the last round.  Ranges of
+// up to LPT_MAX poses.
+constexpr int LPT_MAX = 2048;
+__global__ void __launch_bounds__(1024) lpt_order_kernel(const unsigned* __restrict__ cost, int N,
+                                                         int* __restrict__ order) {
+    __shared__ unsigned long long key[LPT_MAX];
+    const unsigned q = (unsigned)N / 8u, r = (unsigned)N % 8u, x = blockIdx.x;
+    const unsigned start = x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q;
+    const unsigned m = q + (x < r ? 1u : 0u);
+    unsigned P = 1;
+    while (P < m) P <<= 1;
+    for (unsigned t = threadIdx.x; t < P; t += blockDim.x) {
+        // descending cost first, then ascending pose: key = ~cost : pose
+        key[t] = t < m ? ((unsigned long long)(~cost[start + t]) << 32) | (start + t) : ~0ull;
+    }
+    __syncthreads();
+    for (unsigned k = 2; k <= P; k <<= 1)
+        for (unsigned j = k >> 1; j > 0; j >>= 1) {
+            for (unsigned t = threadIdx.x; t < P; t += blockDim.x) {
+                const unsigned u = t ^ j;
+                if (u > t) {
+                    const unsigned long long a = key[t], b = key[u];
+                    if (((t & k) == 0) == (a > b)) { key[t] = b; key[u] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    for (unsigned t = threadIdx.x; t < m; t += blockDim.x) order[start + t] = (int)(key[t] & 0xffffffffu);
+}
+
 // Diagnostic: device transform of every (pose, ligand atom), sorted order.
 __global__ void transform_kernel(DevComplex C, const double* __restrict__ poses, int N,
                                  double* __restrict__ outxyz) {
@@ -1040,6 +1098,11 @@ struct lkl_ctx {
     // and the ligand charges in sorted order that fill their w
     float4* d_lcoords = nullptr;
     size_t cap_lcoords = 0;
+    // per-pose cost and dispatch order of one-tile-per-pose launches
+    unsigned* d_pcost = nullptr;
+    int* d_porder = nullptr;
+    size_t cap_pcost = 0, cap_porder = 0;
+    bool pose_order = true;
     std::vector<float> lq;
     double* d_out = nullptr;
     size_t cap_out = 0;
@@ -1460,7 +1523,7 @@ static void ctx_free(lkl_ctx* c) {
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
-                    c->d_noff_if, c->d_nidx_if, c->d_lcoords};
+                    c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1621,6 +1684,17 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
     if (fail_at("launch")) { set_err("launch_score stage=launch: injected failure"); return -1; }
+    // longest-first dispatch when the launch takes a few rounds of the chip's
+    // workgroup slots: a short pre-pass estimates each pose's cost and orders
+    // each XCD's range (results do not depend on the order)
+    if (c->pose_order && list && nchunks == 1 && !peratom && N > POSE_ORDER_MIN && (N + 7) / 8 <= LPT_MAX) {
+        if (dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
+        if (dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
+        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, c->d_pcost);
+        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3(1024), 0, st, c->d_pcost, N, c->d_porder);
+        HIPCK("launch", hipGetLastError());
+        C.porder = c->d_porder;
+    }
     hipLaunchKernelGGL(fn, dim3((unsigned)(N * nchunks)), dim3((unsigned)c->block), shm, st, C,
                        d_poses, N, d_out, d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_acc,
                        c->d_cnt, chunk, nchunks);
@@ -1722,6 +1796,12 @@ LKL_API int lkl_ctx_score_coords(lkl_ctx* c, const float* coords, int N, unsigne
     HIPCK("readback", hipMemcpy(out, c->d_out, sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToHost));
     if (flags & F_DESOLV)
         HIPCK("readback", hipMemcpy(outS, c->d_outS, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+LKL_API int lkl_ctx_set_pose_order(lkl_ctx* c, int enable) {
+    if (!c) { set_err("lkl_ctx_set_pose_order: null context"); return -1; }
+    c->pose_order = enable != 0;
     return 0;
 }
 

@@ -154,6 +154,10 @@ class Context:
                     "lkl_ctx_score")
         return out[:, 0].copy(), out[:, 1].copy(), s
 
+    def set_pose_order(self, on):
+        """lkl_ctx_set_pose_order: longest-first dispatch of large launches."""
+        _capi.check(lib().lkl_ctx_set_pose_order(self.h, int(bool(on))), "lkl_ctx_set_pose_order")
+
     def score_coords(self, coords, flags):
         """lkl_ctx_score_coords: ligands already placed, coords (N, nl, 3) in
         the descriptor's ligand order -> (E_raw, V, S)."""
