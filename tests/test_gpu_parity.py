@@ -283,6 +283,29 @@ def lib_has_iface(ctx):
     return bool(lklight.lib().lkl_ctx_has_interface(ctx.h))
 
 
+def test_pose_order_is_bitwise_neutral(small):
+    """Longest-first dispatch (launches of 513..16,384 poses) changes only the
+    order workgroups start in: every pose's scores are the same bits with it
+    on and off, with and without desolvation."""
+    cx, orc, _ = small
+    poses = synth.make_poses(1500, cx.meta["rec_axes"], 77)
+    ctx = lklight.Context(cx, desolv=True)
+    for flags, n in ((lklight.family_flags("dna"), 1500),
+                     (lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV, 700)):
+        got = []
+        for on in (1, 0):
+            ctx.set_pose_order(on)
+            got.append(run(ctx, poses[:n], flags))
+        say("pose order on/off, flags", flags, "poses", n)
+        for k in ("e", "v", "s"):
+            if got[0][k] is not None:
+                assert np.array_equal(got[0][k], got[1][k]), k
+    ctx.set_pose_order(1)
+    ref = orc.batch("dna", poses[:600], nthreads=8)
+    assert_scores("dna", run(ctx, poses[:600], lklight.family_flags("dna")), ref)
+    ctx.close()
+
+
 def test_determinism_and_batch_split(small):
     cx, orc, poses = small
     say("determinism (multi-tile poses: accumulators must re-zero between launches)")
