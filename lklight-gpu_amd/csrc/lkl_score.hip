@@ -964,22 +964,40 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
 }
 
 // Expected near-field cost of each pose, for the dispatch order: one wavefront
-// per pose sums the neighbour-list lengths at the nodes of every fourth ligand
-// atom (the walk's work is proportional to the candidates it visits).
-constexpr int COST_STRIDE = 4;
+// per pose sums the neighbour-list lengths at the nodes of every eighth ligand
+// atom (the walk's work is proportional to the candidates it visits).  Each
+// lane places U atoms, then requests their offsets together.
+constexpr int COST_STRIDE = 8;
 __global__ void __launch_bounds__(64) pose_cost_kernel(DevComplex C, const double* __restrict__ poses, int N,
                                                        unsigned* __restrict__ cost) {
     const int pose = blockIdx.x;
     if (pose >= N) return;
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
     const float4* lx = C.lxyzq + (C.lstride ? (size_t)pose * (size_t)C.lstride : 0);
+    constexpr int U = 4;
     unsigned c = 0;
-    for (int j = (int)threadIdx.x * COST_STRIDE; j < C.nl; j += 64 * COST_STRIDE) {
-        const float4 lb = lx[j];
-        double x, y, z;
-        transform(M, lb.x, lb.y, lb.z, x, y, z);
-        const int n = list_node(C, x, y, z);
-        if (n >= 0) c += C.noff[n + 1] - C.noff[n];
+    for (int j0 = (int)threadIdx.x * COST_STRIDE; j0 < C.nl; j0 += U * 64 * COST_STRIDE) {
+        int n[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + u * 64 * COST_STRIDE;
+            n[u] = -1;
+            if (j < C.nl) {
+                const float4 lb = lx[j];
+                double x, y, z;
+                transform(M, lb.x, lb.y, lb.z, x, y, z);
+                n[u] = list_node(C, x, y, z);
+            }
+        }
+        unsigned lo[U], hi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = n[u] >= 0 ? n[u] : 0;
+            lo[u] = C.noff[q];
+            hi[u] = C.noff[q + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) c += n[u] >= 0 ? hi[u] - lo[u] : 0u;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
@@ -1691,7 +1709,9 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         if (dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
         if (dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
         hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, c->d_pcost);
-        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3(1024), 0, st, c->d_pcost, N, c->d_porder);
+        int pm = 64;  // threads of the sort: the padded range, at most 1024
+        while (pm < (N + 7) / 8 && pm < 1024) pm <<= 1;
+        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, c->d_pcost, N, c->d_porder);
         HIPCK("launch", hipGetLastError());
         C.porder = c->d_porder;
     }
