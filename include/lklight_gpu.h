@@ -155,6 +155,8 @@ int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned fla
  * so cheap poses fill the last round of workgroup slots.  Scores are the same
  * bits either way.  0 turns it off (one kernel per launch).  0 / -1. */
 int lkl_ctx_set_pose_order(lkl_ctx* ctx, int enable);
+/* Current setting: 1 / 0, or -1 for a NULL context. */
+int lkl_ctx_pose_order(const lkl_ctx* ctx);
 
 /* Scores ligands already placed: coords N*nl*3 f32 (pose-major, the
  * descriptor's ligand order), one pose per coordinate set, the context's

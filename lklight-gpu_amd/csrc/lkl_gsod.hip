@@ -285,6 +285,15 @@ int lkl_gsod_run(lkl_gsod_buf** pd, lkl_ctx* ctx, int family, unsigned flags, in
     int smax = 0;
     for (int q = 0; q < n_swarms; ++q) smax = std::max(smax, swarm_off[q + 1] - swarm_off[q]);
     const bool listed = smax <= kSwarmList;  // every swarm fits the LDS list
+    // the swarm poses' launches are short and even: the dispatch-order
+    // pre-pass would cost more than it returns, so it is off for the run
+    const int order = lkl_ctx_pose_order(ctx);
+    lkl_ctx_set_pose_order(ctx, 0);
+    struct Restore {
+        lkl_ctx* c;
+        int on;
+        ~Restore() { lkl_ctx_set_pose_order(c, on); }
+    } restore{ctx, order};
     for (int k = 0; k < steps; ++k) {
         if (lkl_ctx_score_async(ctx, G.pose, n, flags, d->out, (flags & LKL_F_DESOLV) ? d->outS : nullptr, st))
             return fail("score", hipSuccess);

@@ -1825,6 +1825,8 @@ LKL_API int lkl_ctx_set_pose_order(lkl_ctx* c, int enable) {
     return 0;
 }
 
+LKL_API int lkl_ctx_pose_order(const lkl_ctx* c) { return c ? (c->pose_order ? 1 : 0) : -1; }
+
 LKL_API int lkl_ctx_build_field(lkl_ctx* c) {
     if (!c) { set_err("lkl_ctx_build_field: null context"); return -1; }
     if (c->nr <= 0 || c->r_ele.empty()) { set_err("lkl_ctx_build_field: receptor charges required"); return -1; }
