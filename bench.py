@@ -94,6 +94,8 @@ def main():
                     help="c2: skip the surrogate-screened GSO comparison")
     ap.add_argument("--no-gso", action="store_true",
                     help="c2: skip the host-driven vs device-resident GSO step comparison")
+    ap.add_argument("--gso-only", action="store_true",
+                    help="c2: only the host-driven vs device-resident GSO step comparison (one JSON line)")
     ap.add_argument("--sur-train", type=int, default=2_000_000, help="c5: training poses")
     ap.add_argument("--sur-epochs", type=int, default=6)
     ap.add_argument("--sur-swarms", type=int, default=40)
@@ -173,6 +175,12 @@ def main():
     n_list, list_dims = ctx.list_info()
     log("near field: %s" % ("neighbour lists, %d entries over %s nodes" % (n_list, list_dims) if n_list
                             else "cell-row walk"))
+    if args.gso_only:
+        res = gso_step_compare(cx, ctx, args.family)
+        print(json.dumps({"metric": "GSO step time, %s, 6 swarms x 200 glowworms (device-resident)" % args.family,
+                          "value": res["device"]["ms_per_step"], "unit": "ms/step", "higher_is_better": False,
+                          "n_gpus": 1, "gso_step": res}))
+        return
     d_poses = torch.from_numpy(poses).to(dev)
     d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
     d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
