@@ -692,9 +692,10 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         // lists after the receptor minima when those live in LDS
         void* lists = DES == DES_LDS ? (void*)(((uintptr_t)(lds_min + C.nr) + 7u) & ~(uintptr_t)7u)
                                      : (void*)lds_min;
-        // desolvation tiles keep one atom per lane (measured faster than the
-        // grouped walk for the 8k x 3k cpydock workload)
-        if constexpr (LIST && DES == DES_NONE)
+        // every launch over lists takes the grouped walk, desolvation included
+        // (its receptor minima are LDS atomics, its ligand minima a shuffle
+        // reduction over the atom's lanes)
+        if constexpr (LIST)
             near_groups<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
         else
             near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
@@ -1679,7 +1680,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         }
         // desolvation tiles hold the whole ligand: order the near field only
         // if its arrays fit (8 bytes of alignment after the receptor minima)
-        const size_t tl = tile_lds((size_t)c->nl, false) + 8;
+        const size_t tl = tile_lds((size_t)c->nl, list) + 8;
         if (want_sort && c->nl <= WORK_TILE_MAX && shm + tl + 1024 <= c->lds_max) shm += tl;
         else C.wgrid = nullptr;
     }
