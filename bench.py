@@ -417,7 +417,7 @@ def surrogate_experiment(args):
     rho = screen.spearman(hp, hs)
     rho_pg = float(np.mean([screen.spearman(hp[i:i + 32], hs[i:i + 32]) for i in range(0, len(hs), 32)]))
     log("held-out spearman %.3f (within-glowworm mean %.3f)" % (rho, rho_pg))
-    # plain exact GSO (the judge)
+    # plain exact GSO (the yardstick the screened run is compared with)
     plain_sc, plain_wall, _ = screen.run_plain(ctx, method, pos, 42, sizes, args.sur_plain_steps)
     ref10 = screen.top_k(plain_sc)
     log("plain GSO %d steps: %.3f s, top-10 %s" % (args.sur_plain_steps, plain_wall, np.round(ref10, 2)))

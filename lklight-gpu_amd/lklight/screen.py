@@ -1,5 +1,5 @@
 """Surrogate-screened GSO: the learned screening step (no reference
-counterpart; the plain exact GSO of lklight.gso stays the fallback and judge).
+counterpart; the plain exact GSO of lklight.gso stays the fallback and the yardstick).
 
 One step, for every glowworm of every swarm:
   1. update_luciferin (scores are already exact from the previous commit);
