@@ -238,6 +238,17 @@ int lkl_ctx_set_phi_packed(lkl_ctx* ctx, int on);
 /* 1 when the context holds the corner-packed far field, else 0. */
 int lkl_ctx_phi_packed(const lkl_ctx* ctx);
 
+/* Receptor parameter types (default on): when the receptor's per-atom
+ * (sqrt(eps), vdw radius, heavy) records take at most 256 distinct values, the
+ * neighbour-list walk copies them and one type byte per atom to LDS and reads
+ * a candidate's parameters there instead of gathering them with its
+ * coordinates.  Scores are bitwise the same either way.  Library-internal
+ * layout with no counterpart in the reference (full_score.cu reads each
+ * receptor atom's parameters from global memory).  lkl_ctx_param_types
+ * returns the number of types in use (0 = off, or more than 256 records). */
+int lkl_ctx_set_param_types(lkl_ctx* ctx, int on);
+int lkl_ctx_param_types(const lkl_ctx* ctx);
+
 /* Neighbour-list state: returns the number of list entries (0 = lists off,
  * -1 = null context) and the node grid dimensions in dims3 (may be NULL). */
 long long lkl_ctx_list_info(const lkl_ctx* ctx, int* dims3);

@@ -75,6 +75,13 @@ struct DevComplex {
     // position in nidx of LIST_PAD entries holding nr, past the last list: a
     // lane whose list has ended loads its index from there
     unsigned ndum;
+    // parameter types (NULL = none): when the receptor's rpar records take at
+    // most 256 distinct values, rtab[ntype] holds them and rtyp holds one type
+    // byte per sorted atom and the dummy, zero-padded to whole 32-bit words;
+    // the list walk then reads the parameters from an LDS copy of both
+    const unsigned* rtyp;
+    const float4* rtab;
+    int ntype;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
     // ligand, spatially sorted (Morton order of the reference coordinates)

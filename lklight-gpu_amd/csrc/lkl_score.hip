@@ -18,6 +18,7 @@
 #include "../../include/lklight_gpu.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -274,6 +275,7 @@ __device__ __forceinline__ void near_any(const DevComplex& C, double x64, double
 // to skip those with none; it never changes a sum.
 constexpr float WORK_H = 2.0f, WORK_PAD = 12.0f;
 constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (LDS arrays)
+constexpr int RTYPE_MAX = 256;        // receptor parameter types of the LDS table (one byte each)
 
 __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz,
                                 int nx, int ny, long long nodes, float r2, unsigned short* __restrict__ out) {
@@ -483,15 +485,18 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 // loads of a subgroup are contiguous, the records of neighbouring atoms'
 // lists (which overlap) share cache lines, and a lane keeps GK gathers in
 // flight.  With desolvation the ligand-side minimum of an atom is reduced over
-// its subgroup by shuffles (exact) and the per-atom term is added once.  All
-// threads of the block must call it (barriers).  LDS: [b u32 x nT][e u32 x
-// nT][active u16 x nT].
+// its subgroup by shuffles (exact) and the per-atom term is added once.  With
+// TYPED the receptor's parameter types (rtab, rtyp) are copied to LDS first and
+// each candidate's parameters are read there (type byte, then table entry)
+// while its record gather is in flight: one gather per candidate instead of two.  All threads of the block must call it
+// (barriers).  LDS: [b u32 x nT][e u32 x nT][active u16 x nT], then, with
+// TYPED, from the next 16-byte boundary [rtab float4 x ntype][rtyp].
 constexpr int GL = 4;        // lanes per list
 constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 constexpr int GK = 8;        // entries per lane per step
 constexpr int GB = 4;        // step-count buckets of the active atoms
 
-template <bool FAR, bool ELEC, bool CLASH, int DES>
+template <bool FAR, bool ELEC, bool CLASH, int DES, bool TYPED = false>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                             unsigned* __restrict__ minR, long long& e, long long& v,
                                             long long& s) {
@@ -503,6 +508,13 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
     unsigned* ab = reinterpret_cast<unsigned*>(lds);
     unsigned* ae = ab + nT;
     unsigned short* act = reinterpret_cast<unsigned short*>(ae + nT);
+    float4* ltab = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds) + ((10u * (unsigned)nT + 15u) & ~15u));
+    unsigned* ltyp = reinterpret_cast<unsigned*>(ltab + (TYPED ? C.ntype : 0));
+    if constexpr (TYPED) {  // visible to the walk after pass 1's barrier
+        for (int k = tid; k < C.ntype; k += bs) ltab[k] = C.rtab[k];
+        const int nwd = (C.nr + 4) >> 2;  // (nr + 1) type bytes in whole words
+        for (int k = tid; k < nwd; k += bs) ltyp[k] = C.rtyp[k];
+    }
     if (tid == 0) next_group = 0u;
     constexpr int P1 = 3;
     for (int t0 = tid; t0 < nT; t0 += P1 * bs) {
@@ -626,7 +638,9 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
 #pragma unroll
             for (int k = 0; k < GK; ++k) {
                 rq[k] = C.rrec[2 * id[k]];
-                rp[k] = C.rrec[2 * id[k] + 1];
+                // with TYPED the two LDS reads complete while the gather is in flight
+                if constexpr (TYPED) rp[k] = ltab[reinterpret_cast<const unsigned char*>(ltyp)[id[k]]];
+                else rp[k] = C.rrec[2 * id[k] + 1];
             }
             int idn[GK];
 #pragma unroll
@@ -659,7 +673,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
 #ifndef LKL_SCORE_ATTR
 #define LKL_SCORE_ATTR
 #endif
-template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM>
+template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM, bool TYPED = false>
 __global__ void __launch_bounds__(512) LKL_SCORE_ATTR
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
@@ -696,7 +710,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         // (its receptor minima are LDS atomics, its ligand minima a shuffle
         // reduction over the atom's lanes)
         if constexpr (LIST)
-            near_groups<FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+            near_groups<FAR, ELEC, CLASH, DES, TYPED>(C, M, j0, j1, lists, minR, e, v, s);
         else
             near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
     } else if constexpr (DES == DES_NONE && !PERATOM) {
@@ -1087,6 +1101,11 @@ struct lkl_ctx {
     int nx = 0, ny = 0, nz = 0;
     double ox = 0, oy = 0, oz = 0, sp = 1;
     float4 *d_rxyzq = nullptr, *d_rpar = nullptr, *d_rrec = nullptr;
+    // receptor parameter types (ntype 0: more than RTYPE_MAX distinct records)
+    unsigned* d_rtyp = nullptr;
+    float4* d_rtab = nullptr;
+    int ntype = 0;
+    bool param_types = true;  // the list walk may read parameters from the LDS type table
     int* d_cstart = nullptr;
     unsigned char* d_rflag = nullptr;
     float *d_rdes = nullptr, *d_rasa = nullptr;
@@ -1372,6 +1391,32 @@ static int build_receptor(lkl_ctx* c) {
         }
         if (dev_upload(&c->d_rrec, rec.data(), rec.size(), c->stream)) return -1;
     }
+    {
+        // parameter types: the distinct rpar records, when there are at most
+        // RTYPE_MAX of them (the dummy takes type 0: it fails every pair test)
+        std::map<std::array<unsigned, 3>, int> ids;
+        std::vector<float4> tab;
+        std::vector<unsigned char> typ(((size_t)nr + 4) / 4 * 4, 0);
+        bool fits = true;
+        for (int p = 0; p < nr && fits; ++p) {
+            std::array<unsigned, 3> key;
+            std::memcpy(key.data(), &par[p], sizeof key);
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                if ((int)tab.size() == RTYPE_MAX) { fits = false; break; }
+                it = ids.emplace(key, (int)tab.size()).first;
+                tab.push_back(par[p]);
+            }
+            typ[p] = (unsigned char)it->second;
+        }
+        c->ntype = 0;
+        if (fits && !tab.empty()) {
+            if (dev_upload(&c->d_rtab, tab.data(), tab.size(), c->stream)) return -1;
+            if (dev_upload(&c->d_rtyp, reinterpret_cast<const unsigned*>(typ.data()), typ.size() / 4, c->stream))
+                return -1;
+            c->ntype = (int)tab.size();
+        }
+    }
     if (dev_upload(&c->d_cstart, start.data(), ncell + 1, c->stream)) return -1;
     if (c->has_desolv) {
         if (dev_upload(&c->d_rflag, flag.data(), (size_t)nr + 1, c->stream)) return -1;
@@ -1476,6 +1521,11 @@ static DevComplex dev_view(const lkl_ctx* c) {
     C.rxyzq = c->d_rxyzq;
     C.rpar = c->d_rpar;
     C.rrec = c->d_rrec;
+    if (c->ntype) {
+        C.rtyp = c->d_rtyp;
+        C.rtab = c->d_rtab;
+        C.ntype = c->ntype;
+    }
     C.cstart = c->d_cstart;
     C.rflag = c->d_rflag;
     C.rdes = c->d_rdes;
@@ -1538,7 +1588,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
 static void ctx_free(lkl_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_cstart, c->d_rflag, c->d_rdes,
+    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_rtyp, c->d_rtab, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
@@ -1626,17 +1676,17 @@ static int reserve_acc(lkl_ctx* c, size_t N) {
     return 0;
 }
 
-template <bool LIST, int DES, bool PA>
+template <bool LIST, int DES, bool PA, bool TY = false>
 static score_fn pick3(unsigned f) {
     switch (f & 7u) {
-        case 0: return score_kernel<LIST, false, false, false, DES, PA>;
-        case 1: return score_kernel<LIST, true, false, false, DES, PA>;
-        case 2: return score_kernel<LIST, false, true, false, DES, PA>;
-        case 3: return score_kernel<LIST, true, true, false, DES, PA>;
-        case 4: return score_kernel<LIST, false, false, true, DES, PA>;
-        case 5: return score_kernel<LIST, true, false, true, DES, PA>;
-        case 6: return score_kernel<LIST, false, true, true, DES, PA>;
-        default: return score_kernel<LIST, true, true, true, DES, PA>;
+        case 0: return score_kernel<LIST, false, false, false, DES, PA, TY>;
+        case 1: return score_kernel<LIST, true, false, false, DES, PA, TY>;
+        case 2: return score_kernel<LIST, false, true, false, DES, PA, TY>;
+        case 3: return score_kernel<LIST, true, true, false, DES, PA, TY>;
+        case 4: return score_kernel<LIST, false, false, true, DES, PA, TY>;
+        case 5: return score_kernel<LIST, true, false, true, DES, PA, TY>;
+        case 6: return score_kernel<LIST, false, true, true, DES, PA, TY>;
+        default: return score_kernel<LIST, true, true, true, DES, PA, TY>;
     }
 }
 
@@ -1651,6 +1701,9 @@ static int validate_flags(const lkl_ctx* c, unsigned flags) {
 // (list bounds and the active list) when grouped, else near_sorted's (index
 // list and bucket ids).
 static size_t tile_lds(size_t nT, bool grouped) { return grouped ? 10 * nT + 16 : 3 * nT + 16; }
+// LDS bytes of the receptor's parameter types beside a grouped tile: the
+// table and one type byte per sorted atom and the dummy, in whole words.
+static size_t type_lds(const lkl_ctx* c) { return 16 * (size_t)c->ntype + ((size_t)c->nr + 4) / 4 * 4; }
 
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
@@ -1697,6 +1750,13 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         const int tmax = (int)std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (list ? 10 : 3));
         if (chunk > tmax) chunk = tmax;  // LDS bound of the tile's arrays
         shm = tile_lds((size_t)chunk, list);
+        // the list walk reads receptor parameters from an LDS type table when
+        // the receptor has one and the table fits beside the tile's arrays
+        const size_t typed = shm + type_lds(c);
+        if (list && !peratom && c->param_types && c->ntype && typed + 1024 <= c->lds_max) {
+            fn = pick3<true, DES_NONE, false, true>(flags);
+            shm = typed;
+        }
     }
     if (peratom) C.wgrid = nullptr;
     const int nchunks = (c->nl + chunk - 1) / chunk;
@@ -2033,6 +2093,14 @@ LKL_API int lkl_ctx_set_phi_packed(lkl_ctx* c, int on) {
 }
 
 LKL_API int lkl_ctx_phi_packed(const lkl_ctx* c) { return c && c->d_phi8 ? 1 : 0; }
+
+LKL_API int lkl_ctx_set_param_types(lkl_ctx* c, int on) {
+    if (!c) { set_err("lkl_ctx_set_param_types: null context"); return -1; }
+    c->param_types = on != 0;
+    return 0;
+}
+
+LKL_API int lkl_ctx_param_types(const lkl_ctx* c) { return c && c->param_types ? c->ntype : 0; }
 
 LKL_API int lkl_ctx_set_list_spacing(lkl_ctx* c, float spacing) {
     if (!c) { set_err("lkl_ctx_set_list_spacing: null context"); return -1; }

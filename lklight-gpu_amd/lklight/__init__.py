@@ -236,6 +236,14 @@ class Context:
     def phi_packed(self):
         return bool(lib().lkl_ctx_phi_packed(self.h))
 
+    def set_param_types(self, on):
+        """lkl_ctx_set_param_types: read receptor parameters from the LDS type table."""
+        _capi.check(lib().lkl_ctx_set_param_types(self.h, int(bool(on))), "lkl_ctx_set_param_types")
+
+    def param_types(self):
+        """Receptor parameter types in use by the list walk (0 = none)."""
+        return int(lib().lkl_ctx_param_types(self.h))
+
     def list_info(self):
         """(list entries, node grid dims); entries 0 = lists off."""
         d = np.zeros(3, np.int32)

@@ -88,6 +88,8 @@ def lib():
         "lkl_ctx_interface": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]),
         "lkl_ctx_interface_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
         "lkl_ctx_phi_packed": (C.c_int, [C.c_void_p]),
+        "lkl_ctx_set_param_types": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_ctx_param_types": (C.c_int, [C.c_void_p]),
         "lkl_ctx_list_info": (C.c_longlong, [C.c_void_p, C.c_void_p]),
         "lkl_batch_score": (C.c_int, [_f, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
                                       C.c_float, C.c_float, _f, _f, _f, _f, _u8, C.c_int,

@@ -117,7 +117,8 @@ def test_cpydock_parity(small):
 def test_tile_and_order_invariance(small):
     """Fixed-point per-pose sums: bitwise identical for every cell edge (cubic
     and narrower along x), block size, ligand tile, neighbour-list spacing (or
-    the cell-row walk), and input atom order."""
+    the cell-row walk), receptor parameters from the LDS type table or with
+    the records, and input atom order."""
     cx, orc, poses = small
     say("tile invariance")
     ctx = lklight.Context(cx, desolv=True)
@@ -132,11 +133,15 @@ def test_tile_and_order_invariance(small):
              (4.0, 128, 256, 0.0, 2.0), (5.0, 64, 128, 0.0, 1.5), (5.0, 256, 0, 1.25, 0.0),
              (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0)]):
         packed = i % 2 == 0
-        say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh} packed phi {packed}")
+        types = i % 4 < 2
+        say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh} packed phi {packed}"
+            f" parameter types {types}")
         ctx.set_tiles(cell, block, chunk, cell_x=cell_x)
         ctx.set_list_spacing(lh)
         ctx.set_phi_packed(packed)
+        ctx.set_param_types(types)
         assert (ctx.list_info()[0] > 0) == (lh > 0) and ctx.phi_packed() == packed
+        assert ctx.param_types() == (20 if types else 0)  # the synthetic receptors have 20 types
         r = run(ctx, poses, flags)
         rc = run(ctx, poses, lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV)
         for k in ("e", "v"):
@@ -281,6 +286,26 @@ def test_interface_counts_match_oracle(small):
 
 def lib_has_iface(ctx):
     return bool(lklight.lib().lkl_ctx_has_interface(ctx.h))
+
+
+def test_param_types_fallback(small):
+    """A receptor with more than 256 distinct parameter records (per-atom vdW
+    radii) has no type table: the list walk gathers each candidate's
+    parameters with its record, bitwise equal to the cell-row walk and within
+    the bar of the oracle."""
+    cx = synth.make_config("small", radii=(1.5, 2.0))
+    cx.set_field(*O.field_build(cx.rec.xyz, cx.rec.ele))
+    poses = synth.make_poses(48, cx.meta["rec_axes"], 9)
+    ctx = lklight.Context(cx)
+    assert ctx.list_info()[0] > 0 and ctx.param_types() == 0
+    flags = lklight.family_flags("dna")
+    got = run(ctx, poses, flags)
+    ctx.set_list_spacing(0.0)
+    row = run(ctx, poses, flags)
+    ctx.close()
+    say("no parameter types: list walk vs cell-row walk, bitwise")
+    assert np.array_equal(got["e"], row["e"]) and np.array_equal(got["v"], row["v"])
+    assert_scores("dna", got, O.Oracle(cx).batch("dna", poses, nthreads=8))
 
 
 def test_pose_order_is_bitwise_neutral(small):
