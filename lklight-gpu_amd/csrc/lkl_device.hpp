@@ -38,7 +38,11 @@ constexpr float SOLV_HI = 40.97f;      // f32 filter of the desolvation window
 constexpr float WIN2 = 100.5f;         // cell window radius^2 (10.025 A, > 10 A + rounding)
 constexpr float SOLV_KEY_T = 40.96f;   // largest float <= SOLV2
 constexpr double MIN_INIT = 10000.0;   // HUGE_DISTANCE of cpydock.rs:592
-constexpr int LIST_PAD = 4;            // neighbour-list entries per aligned 8-byte load
+// neighbour lists are padded to whole blocks of LIST_PAD entries; each block is
+// one step of a 4-lane subgroup of the grouped walk, stored lane-major (entry
+// 4q + r of the block, q < 8, r < 4, at position 8r + q) so a lane loads its 8
+// entries of the step with one 16-byte read
+constexpr int LIST_PAD = 32;
 constexpr float DUMMY_XYZ = 1.0e15f;   // coordinate of the padding record (d2 ~ 3e30, finite)
 
 // Family flag word (full_score.cu:23-25) + desolvation bit.
@@ -66,14 +70,15 @@ struct DevComplex {
     // spacing 1/inv_nh at no + i/inv_nh, the sorted receptor indices of the
     // atoms within rwin + (node half-diagonal) of the node, in sorted order,
     // each list padded to a multiple of LIST_PAD with index nr (a dummy record
-    // far from everything).  noff holds nodes + 1 offsets into nidx.
+    // far from everything) and its blocks stored lane-major.  noff holds
+    // nodes + 1 offsets into nidx.
     const unsigned* noff;
     const unsigned short* nidx;
     // the sorted receptor records interleaved for the list walk: rrec[2p] =
     // rxyzq[p], rrec[2p + 1] = rpar[p] (one 32-byte record per atom, nr + 1)
     const float4* rrec;
-    // position in nidx of LIST_PAD entries holding nr, past the last list: a
-    // lane whose list has ended loads its index from there
+    // position in nidx of a block of LIST_PAD entries holding nr, past the
+    // last list: a lane whose list has ended loads its indices from there
     unsigned ndum;
     // parameter types (NULL = none): when the receptor's rpar records take at
     // most 256 distinct values, rtab[ntype] holds them and rtyp holds one type

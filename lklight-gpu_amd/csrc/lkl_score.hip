@@ -216,7 +216,7 @@ __device__ __forceinline__ void near_list(const DevComplex& C, double x64, doubl
                                           float qj, float svj, float rj, bool hj, bool lf, unsigned b,
                                           unsigned en, long long& e, long long& v, double& minL,
                                           unsigned* __restrict__ minR) {
-    static_assert(LIST_PAD == 4, "one ushort4 per list vector");
+    static_assert(LIST_PAD % 8 == 0, "lists are whole pairs of ushort4 vectors");
     constexpr int U = 2;  // list vectors per step
     const float qjs = qj * E_SCALE_F;
     const float svjs = svj * V_SCALE_F;
@@ -275,6 +275,7 @@ __device__ __forceinline__ void near_any(const DevComplex& C, double x64, double
 // to skip those with none; it never changes a sum.
 constexpr float WORK_H = 2.0f, WORK_PAD = 12.0f;
 constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (LDS arrays)
+constexpr unsigned IFACE_PAD = 4;     // entries per padding unit of the interface pass's lists
 constexpr int RTYPE_MAX = 256;        // receptor parameter types of the LDS table (one byte each)
 
 __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz,
@@ -300,16 +301,24 @@ __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox
 // the brick's box (order-preserving compaction into LDS) and every lane then
 // tests them against its node: |node - r|^2 <= rad2 in f32 (rad carries a
 // 0.01 A margin over the window radius plus the half-diagonal).  COUNT writes
-// each node's list length rounded up to LIST_PAD; FILL writes the indices at
-// the node's offset, in receptor order, then the padding index nr.  Both
-// instantiations evaluate the same test, so they agree on every list.
+// each node's list length rounded up to pad (a power of two); FILL writes the
+// indices at the node's offset, in receptor order, then the padding index nr,
+// with every block of LIST_PAD entries stored lane-major when lanemajor is set.
+// Both instantiations evaluate the same test, so they agree on every list.
 constexpr int NLX = 8, NLY = 8, NLZ = 4, NLT = NLX * NLY * NLZ;
 constexpr int NLCAP = 2048;  // LDS candidate buffer (32 KB)
+
+// position of list entry k in its block of LIST_PAD when blocks are lane-major
+__device__ __forceinline__ unsigned lane_major(unsigned k) {
+    static_assert(LIST_PAD == 32, "blocks of 8 steps x 4 lanes");
+    return (k & ~31u) | ((k & 3u) << 3) | ((k >> 2) & 7u);
+}
 
 template <bool FILL>
 __global__ void __launch_bounds__(NLT)
 nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz, float h, int nx, int ny,
-             int nz, float rad, unsigned* __restrict__ cnt_off, unsigned short* __restrict__ idx) {
+             int nz, float rad, unsigned pad, int lanemajor, unsigned* __restrict__ cnt_off,
+             unsigned short* __restrict__ idx) {
 #pragma clang fp contract(off)
     __shared__ float4 buf[NLCAP];
     __shared__ int wave_cnt[NLT / 64];
@@ -367,7 +376,8 @@ nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float o
                     const float4 b = buf[q];
                     const float dx = px - b.x, dy = py - b.y, dz = pz - b.z;
                     if (dx * dx + dy * dy + dz * dz <= rad2) {
-                        if (FILL && base_out + k < end_out) idx[base_out + k] = (unsigned short)__float_as_int(b.w);
+                        if (FILL && base_out + k < end_out)
+                            idx[base_out + (lanemajor ? lane_major(k) : k)] = (unsigned short)__float_as_int(b.w);
                         ++k;
                     }
                 }
@@ -378,9 +388,9 @@ nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float o
         }
     }
     if (!valid) return;
-    const unsigned padded = (k + (unsigned)LIST_PAD - 1u) / (unsigned)LIST_PAD * (unsigned)LIST_PAD;
+    const unsigned padded = (k + pad - 1u) / pad * pad;
     if (FILL) {
-        for (; base_out + k < end_out; ++k) idx[base_out + k] = (unsigned short)nr;
+        for (; base_out + k < end_out; ++k) idx[base_out + (lanemajor ? lane_major(k) : k)] = (unsigned short)nr;
     } else {
         cnt_off[node] = padded;
     }
@@ -481,16 +491,18 @@ __device__ __forceinline__ void near_sorted(const DevComplex& C, const Mat& M, i
 // their list takes and in (Morton) order within a bucket, and the wavefronts
 // take groups of GA consecutive active atoms from a shared counter.
 // Within a wavefront each GL-lane subgroup walks one atom's list, lane r taking
-// entries r, r + GL, ..., r + (GK-1) GL of each GK*GL-entry step: the index
-// loads of a subgroup are contiguous, the records of neighbouring atoms'
-// lists (which overlap) share cache lines, and a lane keeps GK gathers in
-// flight.  With desolvation the ligand-side minimum of an atom is reduced over
+// entries r, r + GL, ..., r + (GK-1) GL of each GK*GL-entry step (one 16-byte
+// index load from the step's lane-major block): the records that a gather
+// instruction requests for a subgroup are neighbours in the sorted receptor,
+// the records of neighbouring atoms' lists (which overlap) share cache lines,
+// and a lane keeps GK gathers in flight.  With desolvation the ligand-side minimum of an atom is reduced over
 // its subgroup by shuffles (exact) and the per-atom term is added once.  With
 // TYPED the receptor's parameter types (rtab, rtyp) are copied to LDS first and
 // each candidate's parameters are read there (type byte, then table entry)
-// while its record gather is in flight: one gather per candidate instead of two.  All threads of the block must call it
-// (barriers).  LDS: [b u32 x nT][e u32 x nT][active u16 x nT], then, with
-// TYPED, from the next 16-byte boundary [rtab float4 x ntype][rtyp].
+// while its record gather is in flight: one gather per candidate instead of
+// two.  All threads of the block must call it (barriers).  LDS: [b u32 x nT]
+// [e u32 x nT][active u16 x nT], then, with TYPED, from the next 16-byte
+// boundary [rtab float4 x ntype][rtyp].
 constexpr int GL = 4;        // lanes per list
 constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 constexpr int GK = 8;        // entries per lane per step
@@ -625,14 +637,24 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         const float xh = (float)x, yh = (float)y, zh = (float)z;
         const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
         double mloc = MIN_INIT;
-        // indices of the next step are requested before this step's pairs are
-        // formed, so a step waits for its records only
+        // a lane's GK indices of a step are one 16-byte load from the
+        // lane-major block (the dummy block once its list has ended); the next
+        // step's are requested before this step's pairs are formed, so a step
+        // waits for its records only
+        static_assert(GL * GK == LIST_PAD && GK == 8, "one lane-major block per step");
+        auto step_ids = [&](unsigned p, int* out) {
+            const uint4 w = *reinterpret_cast<const uint4*>(C.nidx + (p < en ? p : C.ndum) + (unsigned)(r * GK));
+            out[0] = (int)(w.x & 0xffffu);
+            out[1] = (int)(w.x >> 16);
+            out[2] = (int)(w.y & 0xffffu);
+            out[3] = (int)(w.y >> 16);
+            out[4] = (int)(w.z & 0xffffu);
+            out[5] = (int)(w.z >> 16);
+            out[6] = (int)(w.w & 0xffffu);
+            out[7] = (int)(w.w >> 16);
+        };
         int id[GK];
-#pragma unroll
-        for (int k = 0; k < GK; ++k) {
-            const unsigned pos = b + (unsigned)(k * GL + r);
-            id[k] = (int)C.nidx[pos < en ? pos : C.ndum];
-        }
+        step_ids(b, id);
         for (unsigned p = b; __any(p < en); p += (unsigned)(GK * GL)) {
             float4 rq[GK], rp[GK];
 #pragma unroll
@@ -643,11 +665,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
                 else rp[k] = C.rrec[2 * id[k] + 1];
             }
             int idn[GK];
-#pragma unroll
-            for (int k = 0; k < GK; ++k) {
-                const unsigned pos = p + (unsigned)(GK * GL + k * GL + r);
-                idn[k] = (int)C.nidx[pos < en ? pos : C.ndum];
-            }
+            step_ids(p + (unsigned)(GK * GL), idn);
 #pragma unroll
             for (int k = 0; k < GK; ++k)
                 pair_term<ELEC, CLASH, DES, true>(C, rq[k], id[k], x, y, z, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
@@ -1088,7 +1106,7 @@ struct lkl_ctx {
     int nn[3] = {0, 0, 0};
     float no[3] = {0.f, 0.f, 0.f};
     size_t list_entries = 0;  // 0 = no lists (row walk)
-    unsigned list_dummy = 0;  // position of the LIST_PAD dummy entries in d_nidx
+    unsigned list_dummy = 0;  // position of the block of LIST_PAD dummy entries in d_nidx
     // host copies in reference order (kept to rebuild the cell layout)
     int nr = 0, nl = 0;
     std::vector<float> r_xyz, r_ele, r_svdw, r_vdwr, r_des, r_asa;
@@ -1232,16 +1250,17 @@ static int build_phi8(lkl_ctx* c) {
 constexpr long long LIST_NODES_MAX = 1ll << 27;
 constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of indices
 
-// Count and fill one list set over the node grid (o, n, h) with radius rad;
+// Count and fill one list set over the node grid (o, n, h) with radius rad,
+// each list padded to a multiple of pad (blocks lane-major if lanemajor);
 // outputs a fresh offsets array (nodes + 1) and index array.  Returns the
 // entry count, 0 when it exceeds LIST_ENTRIES_MAX (nothing allocated), or -1.
-static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float h, float rad, unsigned** d_off,
-                            unsigned short** d_idx, unsigned* dummy_pos = nullptr) {
+static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float h, float rad, unsigned pad,
+                            bool lanemajor, unsigned** d_off, unsigned short** d_idx, unsigned* dummy_pos = nullptr) {
     const long long nodes = (long long)n[0] * n[1] * n[2];
     HIPCK("alloc", hipMalloc((void**)d_off, sizeof(unsigned) * (size_t)(nodes + 1)));
     dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
     hipLaunchKernelGGL(nlist_kernel<false>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, *d_off, (unsigned short*)nullptr);
+                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, (unsigned short*)nullptr);
     HIPCK("launch", hipGetLastError());
     HIPCK("sync", hipStreamSynchronize(c->stream));
     std::vector<unsigned> off((size_t)nodes + 1);
@@ -1260,14 +1279,13 @@ static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float 
     }
     off[nodes] = (unsigned)acc;
     HIPCK("upload", h2d(*d_off, off.data(), sizeof(unsigned) * off.size(), c->stream));
-    // LIST_PAD dummy entries (index nr) after the last list
-    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)(acc + LIST_PAD)));
-    const unsigned short dum[LIST_PAD] = {(unsigned short)c->nr, (unsigned short)c->nr, (unsigned short)c->nr,
-                                          (unsigned short)c->nr};
-    HIPCK("upload", h2d(*d_idx + acc, dum, sizeof dum, c->stream));
+    // a block of pad dummy entries (index nr) after the last list
+    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)(acc + pad)));
+    const std::vector<unsigned short> dum(pad, (unsigned short)c->nr);
+    HIPCK("upload", h2d(*d_idx + acc, dum.data(), sizeof(unsigned short) * pad, c->stream));
     if (dummy_pos) *dummy_pos = (unsigned)acc;
     hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, *d_off, *d_idx);
+                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, *d_idx);
     HIPCK("launch", hipGetLastError());
     HIPCK("sync", hipStreamSynchronize(c->stream));
     return (long long)std::max(acc, 1ull);
@@ -1303,7 +1321,7 @@ static int build_lists(lkl_ctx* c) {
         nodes *= n[a];
     }
     if (nodes > LIST_NODES_MAX || (n[1] + NLY - 1) / NLY > 65535 || (n[2] + NLZ - 1) / NLZ > 65535) return 0;
-    const long long entries = fill_lists(c, o, n, h, rad, &c->d_noff, &c->d_nidx, &c->list_dummy);
+    const long long entries = fill_lists(c, o, n, h, rad, LIST_PAD, true, &c->d_noff, &c->d_nidx, &c->list_dummy);
     if (entries <= 0) return (int)entries;
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
     c->list_entries = (size_t)entries;
@@ -1320,7 +1338,7 @@ static int ensure_iface_lists(lkl_ctx* c) {
     free_iface_lists(c);
     const float h = c->list_h;
     const float rad = std::sqrt(15.211f) + 0.5f * std::sqrt(3.0f) * h + 0.01f;
-    const long long entries = fill_lists(c, c->no, c->nn, h, rad, &c->d_noff_if, &c->d_nidx_if);
+    const long long entries = fill_lists(c, c->no, c->nn, h, rad, IFACE_PAD, false, &c->d_noff_if, &c->d_nidx_if);
     if (entries < 0) return -1;
     if (entries == 0) { set_err("interface lists exceed the entry cap"); return -1; }
     c->iflist_version = c->version;

@@ -175,7 +175,8 @@ def test_tile_and_order_invariance(small):
 def _list_entries_host(xyz, h):
     """Neighbour-list size by brute force in f32: per node of the list grid, the
     receptor atoms within rad = sqrt(100.5) + h*sqrt(3)/2 + 0.01 of the node,
-    padded to a multiple of 4 (the library's list layout)."""
+    padded to a multiple of 32 (the library's list layout: whole blocks of one
+    step of the grouped walk)."""
     f = np.float32
     rad = f(np.sqrt(f(100.5))) + f(0.5) * np.sqrt(f(3.0)) * f(h) + f(0.01)
     lo, hi = xyz.min(0), xyz.max(0)
@@ -191,7 +192,7 @@ def _list_entries_host(xyz, h):
             dx = ax[0][:, None] - r[None, :, 0]
             d2 = (dx * dx + dy2[None, :]) + dz2[None, :]
             c = (d2 <= rad * rad).sum(1)
-            total += int(((c + 3) // 4 * 4).sum())
+            total += int(((c + 31) // 32 * 32).sum())
     return total, tuple(int(v) for v in n)
 
 
@@ -206,7 +207,7 @@ def test_list_build_counts(gpu):
         got, dims = ctx.list_info()
         want, wdims = _list_entries_host(cx.rec.xyz, h)
         assert dims == wdims, (h, dims, wdims)
-        assert abs(got - want) <= 8, (h, got, want)  # a pair exactly on the f32 boundary may flip
+        assert abs(got - want) <= 64, (h, got, want)  # a pair exactly on the f32 boundary may flip a block
     ctx.set_list_spacing(0.0)
     assert ctx.list_info()[0] == 0
     with pytest.raises(lklight.GpuError, match="list spacing"):
