@@ -659,8 +659,9 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
             float4 rq[GK], rp[GK];
 #pragma unroll
             for (int k = 0; k < GK; ++k) {
-                rq[k] = C.rrec[2 * id[k]];
-                // with TYPED the two LDS reads complete while the gather is in flight
+                // with TYPED only the coordinates are gathered, from the dense
+                // (x, y, z, q) array, and the two LDS reads complete meanwhile
+                rq[k] = TYPED ? C.rxyzq[id[k]] : C.rrec[2 * id[k]];
                 if constexpr (TYPED) rp[k] = ltab[reinterpret_cast<const unsigned char*>(ltyp)[id[k]]];
                 else rp[k] = C.rrec[2 * id[k] + 1];
             }
@@ -811,15 +812,18 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
 
 // ── multi-problem batch ───────────────────────────────────────────────────────
 // Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
-// thread's share (no desolvation, no per-atom output).
+// thread's share (no desolvation, no per-atom output).  A problem whose view
+// carries parameter types (ntype > 0; the host clears it when the table does
+// not fit the launch's LDS) walks its lists with them.
 template <bool LIST, bool FAR, bool ELEC, bool CLASH>
 __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
     if (C.wgrid) {
         long long s_unused = 0;
-        if constexpr (LIST)
-            near_groups<FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
-        else
+        if constexpr (LIST) {
+            if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+            else near_groups<FAR, ELEC, CLASH, DES_NONE, false>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+        } else
             near_sorted<false, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
     }
@@ -2300,7 +2304,15 @@ static int batch_refresh(lkl_multi* b) {
         if (cx[r].wgrid) {  // LDS bound of the work-ordered tile's arrays
             const size_t tmax = std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (b->list ? 10 : 3));
             chunk = std::min(chunk, (int)tmax);
-            b->lds = std::max(b->lds, tile_lds((size_t)chunk, b->list));
+            size_t need = tile_lds((size_t)chunk, b->list);
+            // the parameter-type table beside the tile's arrays, when it fits
+            if (b->list && c->param_types && c->ntype && need + type_lds(c) + 1024 <= c->lds_max)
+                need += type_lds(c);
+            else
+                cx[r].ntype = 0;
+            b->lds = std::max(b->lds, need);
+        } else {
+            cx[r].ntype = 0;
         }
         pd[r].tile_begin = tile;
         pd[r].nchunks = (c->nl + chunk - 1) / chunk;
