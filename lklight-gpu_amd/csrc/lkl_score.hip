@@ -1043,9 +1043,9 @@ __global__ void __launch_bounds__(64) pose_cost_kernel(DevComplex C, const doubl
 
 // Longest-first order within each XCD's range of tiles (xcd_tile gives XCD x
 // the contiguous range it dispatches in ascending order): block x sorts its
-// range by cost, descending, ties by pose, with a bitonic sort in LDS, so the
-// costly poses start first and the cheap ones fill the last round.  Ranges of
-// up to LPT_MAX poses.
+// range by cost, descending, ties by pose (a rank sort in LDS), so the costly
+// poses start first and the cheap ones fill the last round.  Ranges of up to
+// LPT_MAX poses.
 constexpr int LPT_MAX = 2048;
 __global__ void __launch_bounds__(1024) lpt_order_kernel(const unsigned* __restrict__ cost, int N,
                                                          int* __restrict__ order) {
@@ -1053,25 +1053,19 @@ __global__ void __launch_bounds__(1024) lpt_order_kernel(const unsigned* __restr
     const unsigned q = (unsigned)N / 8u, r = (unsigned)N % 8u, x = blockIdx.x;
     const unsigned start = x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q;
     const unsigned m = q + (x < r ? 1u : 0u);
-    unsigned P = 1;
-    while (P < m) P <<= 1;
-    for (unsigned t = threadIdx.x; t < P; t += blockDim.x) {
-        // descending cost first, then ascending pose: key = ~cost : pose
-        key[t] = t < m ? ((unsigned long long)(~cost[start + t]) << 32) | (start + t) : ~0ull;
+    for (unsigned t = threadIdx.x; t < m; t += blockDim.x) {
+        // descending cost first, then ascending pose: key = ~cost : pose (unique)
+        key[t] = ((unsigned long long)(~cost[start + t]) << 32) | (start + t);
     }
     __syncthreads();
-    for (unsigned k = 2; k <= P; k <<= 1)
-        for (unsigned j = k >> 1; j > 0; j >>= 1) {
-            for (unsigned t = threadIdx.x; t < P; t += blockDim.x) {
-                const unsigned u = t ^ j;
-                if (u > t) {
-                    const unsigned long long a = key[t], b = key[u];
-                    if (((t & k) == 0) == (a > b)) { key[t] = b; key[u] = a; }
-                }
-            }
-            __syncthreads();
-        }
-    for (unsigned t = threadIdx.x; t < m; t += blockDim.x) order[start + t] = (int)(key[t] & 0xffffffffu);
+    // rank sort: a key's position is the number of smaller keys (every lane
+    // reads the same key at once, an LDS broadcast; one barrier in all)
+    for (unsigned t = threadIdx.x; t < m; t += blockDim.x) {
+        const unsigned long long k = key[t];
+        unsigned rank = 0;
+        for (unsigned i = 0; i < m; ++i) rank += key[i] < k ? 1u : 0u;
+        order[start + rank] = (int)(k & 0xffffffffu);
+    }
 }
 
 // Diagnostic: device transform of every (pose, ligand atom), sorted order.
@@ -1792,7 +1786,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         if (dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
         if (dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
         hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, c->d_pcost);
-        int pm = 64;  // threads of the sort: the padded range, at most 1024
+        int pm = 64;  // threads of the sort: the range rounded up to a power of two, at most 1024
         while (pm < (N + 7) / 8 && pm < 1024) pm <<= 1;
         hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, c->d_pcost, N, c->d_porder);
         HIPCK("launch", hipGetLastError());
