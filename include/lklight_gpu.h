@@ -149,10 +149,11 @@ int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, doub
 int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
                         double* d_out, double* d_outS, void* stream);
 
-/* Longest-first dispatch (default on): one-tile-per-pose launches of 513 to
+/* Longest-first dispatch (default on): one-tile-per-pose launches of 257 to
  * 16,384 poses over neighbour lists first estimate each pose's near-field
  * cost (list lengths at its atoms' nodes) and start the costly poses first,
- * so cheap poses fill the last round of workgroup slots.  Scores are the same
+ * so costly tiles share CUs with cheap ones and cheap poses fill the last
+ * round of workgroup slots.  Scores are the same
  * bits either way.  0 turns it off (one kernel per launch).  0 / -1. */
 int lkl_ctx_set_pose_order(lkl_ctx* ctx, int enable);
 /* Current setting: 1 / 0, or -1 for a NULL context. */
@@ -176,9 +177,9 @@ int lkl_dataset_poses(unsigned long long seed, int n, double* out);
  * order of the descriptor), computed by the device transform. */
 int lkl_ctx_transform(lkl_ctx* ctx, const double* poses, int N, double* out_xyz);
 
-/* Set the receptor cell edge (A, in (0.5, 20]), workgroup size (64, 128, 256
- * or 512) and ligand atoms per workgroup tile (0 = the whole ligand) used by
- * later score calls.  Scores are bitwise identical for every setting. */
+/* Set the receptor cell edge (A, in (0.5, 20]), workgroup size (64, 128, 256,
+ * 512 or 1024) and ligand atoms per workgroup tile (0 = the whole ligand) used
+ * by later score calls.  Scores are bitwise identical for every setting. */
 int lkl_ctx_set_tiles(lkl_ctx* ctx, float cell_size, int block, int chunk);
 
 /* Receptor cell edge along x (A, in (0.5, 20]; 0 = cell_size, cubic cells).
@@ -256,7 +257,7 @@ long long lkl_ctx_list_info(const lkl_ctx* ctx, int* dims3);
 /* Autotuner: time every (cell edge y/z, cell edge x, workgroup size, ligand
  * tile) candidate on N host poses with `flags`, keep the fastest in the context
  * and report it.  Scores are bitwise identical for every candidate.  table
- * (may be NULL) receives up to 256 rows of (cell A, cell_x A, block, chunk,
+ * (may be NULL) receives up to 320 rows of (cell A, cell_x A, block, chunk,
  * median ms); *n_table their count.  No reference counterpart
  * (full_score.cu:442 hard-codes one 256-thread shape). */
 int lkl_ctx_autotune(lkl_ctx* ctx, const double* poses, int N, unsigned flags,

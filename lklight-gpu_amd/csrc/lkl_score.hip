@@ -86,8 +86,10 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 #define SQRT_D(x) __builtin_amdgcn_sqrtf(x)
 #endif
 constexpr int NU = LKL_NU;
-// one-tile-per-pose launches above this many poses are dispatched longest first
-constexpr int POSE_ORDER_MIN = 512;
+// one-tile-per-pose launches above this many poses are dispatched longest
+// first: beyond one tile per CU, tiles share CUs (and, further on, wait for a
+// free slot), and starting the costly ones first pairs them with cheap ones
+constexpr int POSE_ORDER_MIN = 256;
 
 // One receptor candidate r (sorted index p) against ligand atom (x64, y64,
 // z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
@@ -692,8 +694,11 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
 #ifndef LKL_SCORE_ATTR
 #define LKL_SCORE_ATTR
 #endif
+// Launch bounds of 1024 threads also cap every variant at 128 VGPRs, so two
+// 512-thread tiles always share a CU (the desolvation variants would otherwise
+// take up to 148 and run one tile per CU).
 template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM, bool TYPED = false>
-__global__ void __launch_bounds__(512) LKL_SCORE_ATTR
+__global__ void __launch_bounds__(1024) LKL_SCORE_ATTR
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
              float* __restrict__ pa_v, long long* __restrict__ acc, unsigned* __restrict__ cnt,
@@ -1956,8 +1961,8 @@ LKL_API int lkl_ctx_transform(lkl_ctx* c, const double* poses, int N, double* ou
 LKL_API int lkl_ctx_set_tiles(lkl_ctx* c, float cell_size, int block, int chunk) {
     if (!c) { set_err("lkl_ctx_set_tiles: null context"); return -1; }
     if (!(cell_size > 0.5f && cell_size <= 20.0f)) { set_err("cell_size %.3f out of (0.5, 20]", cell_size); return -1; }
-    if (block != 64 && block != 128 && block != 256 && block != 512) {
-        set_err("block %d not in {64,128,256,512}", block);
+    if (block != 64 && block != 128 && block != 256 && block != 512 && block != 1024) {
+        set_err("block %d not in {64,128,256,512,1024}", block);
         return -1;
     }
     if (chunk < 0) { set_err("chunk %d must be >= 0", chunk); return -1; }
@@ -2158,9 +2163,10 @@ LKL_API int lkl_ctx_set_cell_x(lkl_ctx* c, float cell_x) {
 // (cell, cell_x, block, chunk, median ms).
 static const float k_cells[] = {3.4f, 4.0f, 5.0f, 6.0f, 7.5f};
 static const float k_cell_x[] = {0.0f, 2.5f, 1.25f};  // 0 = cubic cells
-static const int k_blocks[] = {64, 128, 256, 512};
+static const int k_blocks[] = {64, 128, 256, 512};            // multi_kernel (launch bounds 512)
+static const int k_ctx_blocks[] = {64, 128, 256, 512, 1024};  // score_kernel
 static const int k_chunk_mult[] = {1, 2, 4, 0};  // ligand atoms per lane (0 = whole ligand)
-constexpr int TUNE_ROWS = 256;
+constexpr int TUNE_ROWS = 320;
 
 LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned flags,
                              float* best_cell, float* best_cell_x, int* best_block, int* best_chunk,
@@ -2186,7 +2192,7 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
                 c->cell_x = cx;
                 if (build_receptor(c)) { rc = -1; break; }
             }
-            for (int block : k_blocks) {
+            for (int block : k_ctx_blocks) {
                 for (int mult : k_chunk_mult) {
                     const int chunk = mult ? block * mult : 0;
                     if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile

@@ -192,7 +192,7 @@ class Context:
         table[(cell, cell_x, block, chunk, ms)])."""
         p = _capi.as_f64(poses).reshape(-1, 7)
         cell, cx, block, chunk, n = C.c_float(0), C.c_float(0), C.c_int(0), C.c_int(0), C.c_int(0)
-        table = np.zeros((256, 5), np.float32)
+        table = np.zeros((320, 5), np.float32)
         _capi.check(lib().lkl_ctx_autotune(self.h, _capi.ptr(p, C.c_double), p.shape[0], flags,
                                            C.byref(cell), C.byref(cx), C.byref(block), C.byref(chunk),
                                            _capi.ptr(table, C.c_float), C.byref(n)),
