@@ -284,8 +284,8 @@ lkl_multi* lkl_multi_create(lkl_ctx* const* ctxs, const unsigned* flags, const i
 void lkl_multi_destroy(lkl_multi* m);
 /* Total poses P (the batch rows: problem 0's poses first). */
 int lkl_multi_poses(const lkl_multi* m);
-/* Launch shape: block in {64,128,256,512}, ligand atoms per lane (0 = whole
- * ligand per tile). */
+/* Launch shape: block in {64,128,256,512,1024}, ligand atoms per lane (0 =
+ * whole ligand per tile). */
 int lkl_multi_set_tiles(lkl_multi* m, int block, int chunk_mult);
 /* Host buffers poses (P x 7) -> out (P x 2 of E_raw, V).  One step is H2D,
  * the batch kernel, D2H; with the graph on (default) the three are captured

@@ -862,7 +862,7 @@ struct ProblemDesc {
 // with a divergent tid==0 claim was restructured by the compiler into nested
 // loops that cycle the barriers without the claiming lane, which never ends.)
 template <bool LIST>
-__global__ void __launch_bounds__(512)
+__global__ void __launch_bounds__(1024)
 multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__ probs, int K,
              int total_tiles, const double* __restrict__ poses, double* __restrict__ out,
              long long* __restrict__ acc, unsigned* __restrict__ cnt) {
@@ -2163,8 +2163,7 @@ LKL_API int lkl_ctx_set_cell_x(lkl_ctx* c, float cell_x) {
 // (cell, cell_x, block, chunk, median ms).
 static const float k_cells[] = {3.4f, 4.0f, 5.0f, 6.0f, 7.5f};
 static const float k_cell_x[] = {0.0f, 2.5f, 1.25f};  // 0 = cubic cells
-static const int k_blocks[] = {64, 128, 256, 512};            // multi_kernel (launch bounds 512)
-static const int k_ctx_blocks[] = {64, 128, 256, 512, 1024};  // score_kernel
+static const int k_blocks[] = {64, 128, 256, 512, 1024};  // workgroup sizes the autotuners try
 static const int k_chunk_mult[] = {1, 2, 4, 0};  // ligand atoms per lane (0 = whole ligand)
 constexpr int TUNE_ROWS = 320;
 
@@ -2192,7 +2191,7 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
                 c->cell_x = cx;
                 if (build_receptor(c)) { rc = -1; break; }
             }
-            for (int block : k_ctx_blocks) {
+            for (int block : k_blocks) {
                 for (int mult : k_chunk_mult) {
                     const int chunk = mult ? block * mult : 0;
                     if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile
@@ -2406,7 +2405,7 @@ LKL_API int lkl_multi_poses(const lkl_multi* b) { return b ? b->P : -1; }
 
 LKL_API int lkl_multi_set_tiles(lkl_multi* b, int block, int chunk_mult) {
     if (!b) { set_err("lkl_multi_set_tiles: null batch"); return -1; }
-    if ((block != 64 && block != 128 && block != 256 && block != 512) || chunk_mult < 0) {
+    if ((block != 64 && block != 128 && block != 256 && block != 512 && block != 1024) || chunk_mult < 0) {
         set_err("lkl_multi_set_tiles: block %d / chunk_mult %d out of range", block, chunk_mult);
         return -1;
     }
