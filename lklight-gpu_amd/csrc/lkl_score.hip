@@ -514,7 +514,6 @@ template <bool FAR, bool ELEC, bool CLASH, int DES, bool TYPED = false>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                             unsigned* __restrict__ minR, long long& e, long long& v,
                                             long long& s) {
-    __shared__ unsigned wtot[16];
     __shared__ unsigned next_group;
     const int tid = (int)threadIdx.x, bs = (int)blockDim.x;
     const int lane = tid & 63, w = tid >> 6, nw = (bs + 63) >> 6;
@@ -618,7 +617,6 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
     }
     const unsigned ngroups = (nact + (unsigned)GA - 1u) / (unsigned)GA;
     const int sub = lane / GL, r = lane % GL;
-    const unsigned short dn = (unsigned short)C.nr;  // the dummy record
     for (;;) {
         unsigned g = 0;
         if (lane == 0) g = atomicAdd(&next_group, 1u);
@@ -645,15 +643,15 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         // waits for its records only
         static_assert(GL * GK == LIST_PAD && GK == 8, "one lane-major block per step");
         auto step_ids = [&](unsigned p, int* out) {
-            const uint4 w = *reinterpret_cast<const uint4*>(C.nidx + (p < en ? p : C.ndum) + (unsigned)(r * GK));
-            out[0] = (int)(w.x & 0xffffu);
-            out[1] = (int)(w.x >> 16);
-            out[2] = (int)(w.y & 0xffffu);
-            out[3] = (int)(w.y >> 16);
-            out[4] = (int)(w.z & 0xffffu);
-            out[5] = (int)(w.z >> 16);
-            out[6] = (int)(w.w & 0xffffu);
-            out[7] = (int)(w.w >> 16);
+            const uint4 q = *reinterpret_cast<const uint4*>(C.nidx + (p < en ? p : C.ndum) + (unsigned)(r * GK));
+            out[0] = (int)(q.x & 0xffffu);
+            out[1] = (int)(q.x >> 16);
+            out[2] = (int)(q.y & 0xffffu);
+            out[3] = (int)(q.y >> 16);
+            out[4] = (int)(q.z & 0xffffu);
+            out[5] = (int)(q.z >> 16);
+            out[6] = (int)(q.w & 0xffffu);
+            out[7] = (int)(q.w >> 16);
         };
         int id[GK];
         step_ids(b, id);
