@@ -1,19 +1,28 @@
 """Benchmark of the batched GSO pose-scoring path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--family dna] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--only c2-dna,...] [--no-cpu]
 
-Workload (one step = one scoring launch of one GSO step of 6 swarms x 200
-glowworms): synthetic 12,625-atom receptor (RSA, 3.0 A separation, (60, 55, 50) A
-ellipsoid) x 1,500-atom ligand, 1,200 poses per step, dna flags (far field +
-electrostatics + clash), FP32 pair terms.  Poses for all steps are resident in
-HBM before the timed region; each step scores a fresh 1,200-pose slice into a
-device buffer.  Prints one JSON line.
+Headline workload (one step = one blocking scoring call of one GSO step of
+6 swarms x 200 glowworms): synthetic 12,625-atom receptor (RSA, 3.0 A
+separation, (60, 55, 50) A ellipsoid) x 1,500-atom ligand, 1,200 poses per
+step, dna flags (far field + electrostatics + clash), FP32 pair terms.  Each
+step is lkl_ctx_score on host buffers: poses H2D, kernels, scores D2H, so
+`value` is the steady-state rate of the whole scoring call.  The rate with the
+poses already resident in HBM is reported beside it.
+
+The same line carries one object per further workload (`configs`): the same
+complex under the pydock and vdw families, the 8,000 x 3,000-atom cpydock
+workload and the 64-complex batch, each with its roofline, CPU baseline and
+maximum error against the oracle.  Prints one JSON line.
 """
 import argparse
+import ctypes as C
 import json
-import math
 import os
+import statistics
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -22,7 +31,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "pose-evals/s (dna, 12.6k-atom rec); surrogate/plain speedup at equal top-10 E"
 HBM_PEAK_GBS = 8000.0
-POSES_PER_STEP = 1200
+TRAFFIC_FILE = "profiles/score_kernel_counters.json"
+ALL_CONFIGS = ("c2-dna", "c2-pydock", "c2-vdw", "c3-cpydock", "c4")
+TIME_EVERY = 4  # kernel events on every 4th timed call
 
 
 def log(*a):
@@ -32,44 +43,407 @@ def log(*a):
 def pose_bytes(nl, n_far, n_near, desolv=False, nr=0):
     """Algorithmic bytes of one pose evaluation (no-reuse gather model):
     pose 7 f64 + out 2 f64 + ligand 25 B/atom + 8 phi corners per charged atom
-    sampled in the grid + one 25 B receptor record per pair within 10 A."""
+    sampled in the grid + one 25 B receptor record per pair within 10 A
+    (+ asa and des per atom with desolvation)."""
     b = 56 + 16 + 25 * nl + 32 * n_far + 25 * n_near
     if desolv:
         b += 8 * (nr + nl)
     return b
 
 
-TRAFFIC_FILE = "profiles/score_kernel_c2_dna_counters.json"
-
-
-def measured_traffic(workload, family, n):
+def measured_traffic(key):
     """HBM bytes per launch of the scoring kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE runs (scaled by the copy-kernel calibration), for
-    the workload they were taken on; None otherwise."""
+    exactly this workload, family, poses per launch and launch shape; None
+    otherwise."""
     try:
         with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
             d = json.load(f)
     except (OSError, ValueError):
         return None
-    if d.get("workload") != "%s-%s" % (workload, family) or d.get("poses_per_launch") != n:
-        return None
-    return d["traffic_bytes_per_launch"]
+    for e in d.get("entries", []):
+        if all(e.get(k) == v for k, v in key.items()):
+            return e["traffic_bytes_per_launch"]
+    return None
 
 
-def cpu_baseline(cx, poses, method, threads, budget_s=15.0):
-    """The oracle (C restatement of the reference CPU grid path, f64) on the
-    host cores, parallel over poses like the reference's rayon loop."""
+def host_info():
+    """Host cores this process may use (affinity, cgroup quota) and the CPU model."""
+    nproc = os.cpu_count() or 1
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = int(q) / int(per)
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                model = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"nproc": nproc, "affinity_cpus": aff, "cgroup_cpu_quota": quota, "model": model,
+            "threads": threads}
+
+
+class CpuLeg:
+    """The oracle (f64 C restatement of the reference CPU grid path, parallel
+    over poses like the reference's rayon loop) timed on the host: all cores
+    this process may use at -O3 -march=x86-64 (the reference's ISA,
+    .cargo/config.toml:1-2), the same at -march=native (built here, on this
+    host), and one thread.  Each figure is the median of repeated runs over
+    the same pose sample after a warm-up."""
+
+    def __init__(self, info, enabled=True):
+        self.info, self.enabled = info, enabled
+        self.native = None
+        if enabled:
+            try:
+                from oracle import lkl_oracle as O
+                self.native = O.build_native(tempfile.mkdtemp(prefix="lkl_oracle_native_"))
+            except (OSError, subprocess.SubprocessError) as e:
+                log("native oracle build failed: %s" % e)
+
+    @staticmethod
+    def _rate(orc, method, poses, threads, reps):
+        rates = []
+        r = None
+        for _ in range(reps):
+            t = time.perf_counter()
+            r = orc.batch(method, poses, nthreads=threads, counts=True)
+            rates.append(poses.shape[0] / (time.perf_counter() - t))
+        return statistics.median(rates), rates, r
+
+    def run(self, cx, method, poses, budget_s, label):
+        """-> (cpu_baseline dict, oracle results on the sample poses)."""
+        from oracle import lkl_oracle as O
+        th = self.info["threads"]
+        orc = O.Oracle(cx)
+        n0 = min(poses.shape[0], max(2 * th, 32))
+        t = time.perf_counter()
+        orc.batch(method, poses[:n0], nthreads=th)  # warm-up + size estimate
+        est = n0 / max(time.perf_counter() - t, 1e-4)
+        reps = 5
+        n = int(min(poses.shape[0], max(n0, est * budget_s * 0.5 / reps)))
+        sample = poses[:n]
+        med, rates, r = self._rate(orc, method, sample, th, reps)
+        out = {"value": med, "unit": "pose-evals/s", "cores": th, "kind": "port",
+               "sample": "%s: %d poses of the timed workload, median of %d runs (%s), %s host, "
+                         "f64 C restatement of the reference CPU grid path, -O3 -march=x86-64"
+                         % (label, n, reps, ", ".join("%.0f" % x for x in rates), self.info["model"]),
+               "host": self.info}
+        if self.native:
+            orc_n = O.Oracle(cx, lib_path=self.native)
+            orc_n.batch(method, sample[:n0], nthreads=th)
+            med_n, rates_n, _ = self._rate(orc_n, method, sample, th, 3)
+            out["march_native"] = {"value": med_n, "cores": th, "runs": rates_n}
+        n1 = int(min(n, max(8, med / th * budget_s * 0.2 / 3)))
+        med_1, rates_1, _ = self._rate(orc, method, sample[:n1], 1, 3)
+        out["single_thread"] = {"value": med_1, "cores": 1, "poses": n1, "runs": rates_1}
+        return out, r
+
+
+def rel_err(method, g_e, g_v, g_s, r):
+    import lklight
+    import numpy as np
+    n = r["e"].shape[0]
+    a = lklight.fold_scores(method, g_e[:n], g_v[:n], None if g_s is None else g_s[:n])
+    b = lklight.fold_scores(method, r["e"], r["v"], r["s"] if method == "cpydock" else None)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1.0)))
+
+
+def set_tiles(ctx, key, poses, flags, tune):
+    """Apply the stored autotuner choice for `key` (or run the autotuner)."""
+    import lklight
+    if tune:
+        cell, cell_x, block, chunk, table = ctx.autotune(poses, flags)
+        t = {"cell": cell, "cell_x": cell_x, "block": block, "chunk": chunk, "source": "autotuned now"}
+        t["table_best_ms"] = min(row[4] for row in table)
+        return t
+    t = dict(lklight.tuned_tiles(key))
+    ctx.set_tiles(t["cell"], t["block"], t["chunk"], cell_x=t["cell_x"])
+    t["source"] = "lklight-gpu_amd/lklight/tiles.json"
+    return t
+
+
+def bench_single(key, cx, K, W, args, cpu, cpu_budget):
+    """One workload on one context: K blocking host-buffer scoring calls (the
+    metric), the same poses resident in HBM, the CPU baseline and the error."""
+    import numpy as np
+    import torch
+    import lklight
+    from lklight import _capi, synth
+
+    workload, method = key.split("-")
+    desolv = method == "cpydock"
+    ctx = lklight.Context(cx, desolv=desolv, with_field=method != "vdw")
+    flags = lklight.family_flags(method) | (lklight.F_DESOLV if desolv else 0)
+    N = 400 if workload == "c3" else 1200
+    poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
+    tiles = set_tiles(ctx, key, poses[:N], flags, args.tune)
+    n_list, list_dims = ctx.list_info()
+    ctx.set_timing(True)  # creates the events; switched on per timed call
+    ctx.set_timing(False)
+
+    # the metric: blocking C-ABI calls on host buffers (H2D poses, kernels,
+    # D2H scores), made as a compiled host makes them: preallocated buffers,
+    # one lkl_ctx_score per step.  Every TIME_EVERY-th call also records HIP
+    # events around its kernels (about 5 us per event on the call's stream).
+    L = _capi.lib()
+    host = [np.ascontiguousarray(poses[i * N:(i + 1) * N]) for i in range(K + W)]
+    outs = np.zeros((K + W, N, 2))
+    outS = np.zeros((K + W, N))
+    p_in = [_capi.ptr(h, C.c_double) for h in host]
+    p_out = [_capi.ptr(outs[i], C.c_double) for i in range(K + W)]
+    p_s = [_capi.ptr(outS[i], C.c_double) if desolv else None for i in range(K + W)]
+
+    def call(i):
+        _capi.check(L.lkl_ctx_score(ctx.h, p_in[i], N, flags, p_out[i], p_s[i]), "lkl_ctx_score")
+    for i in range(W):
+        call(i)
+    pre_ms, ker_ms = [], []
+    t0 = time.perf_counter()
+    for k in range(K):
+        timed = k % TIME_EVERY == 0
+        if timed:
+            L.lkl_ctx_set_timing(ctx.h, 1)
+        call(W + k)
+        if timed:
+            a, b = ctx.kernel_ms()
+            pre_ms.append(a)
+            ker_ms.append(b)
+            L.lkl_ctx_set_timing(ctx.h, 0)
+    wall = time.perf_counter() - t0
+    value = N * K / wall
+    g_e, g_v = outs[W:, :, 0].reshape(-1), outs[W:, :, 1].reshape(-1)
+    g_s = outS[W:].reshape(-1) if desolv else None
+    # steady state over a longer run of the same calls (no events)
+    reps = args.steady
+    steady = None
+    if reps > 0:
+        t0 = time.perf_counter()
+        for k in range(reps):
+            call(W + k % K)
+        steady = N * reps / (time.perf_counter() - t0)
+
+    # the same steps with the poses resident in HBM (no copies)
+    dev = torch.device("cuda:0")
+    d_poses = torch.from_numpy(poses).to(dev)
+    d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
+    d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+
+    def step(i):
+        ctx.score_async(d_poses[i * N:(i + 1) * N].data_ptr(), N, flags, d_out[i].data_ptr(),
+                        d_s[i].data_ptr() if desolv else None, sh)
+    for i in range(W):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(K):
+        step(W + k)
+    torch.cuda.synchronize()
+    resident = N * K / (time.perf_counter() - t0)
+    same = bool(np.array_equal(d_out[W:].reshape(-1, 2)[:, 0].cpu().numpy(), g_e)
+                and np.array_equal(d_out[W:].reshape(-1, 2)[:, 1].cpu().numpy(), g_v))
+    log("%s: host-buffer calls %.0f pose-evals/s (steady %s), resident %.0f, score kernel %.4f ms"
+        % (key, value, "%.0f" % steady if steady else "-", resident, sum(ker_ms) / len(ker_ms)))
+
+    cpu_b, r = (None, None)
+    if cpu.enabled:
+        cpu_b, r = cpu.run(cx, method, poses[W * N:(W + K) * N], cpu_budget, key)
+        cpu_b["gpu_over_cpu"] = value / cpu_b["value"]
+        cpu_b["max_rel_err_vs_cpu"] = rel_err(method, g_e, g_v, g_s, r)
+        log("%s: cpu %.0f pose-evals/s on %d threads (native %s, 1 thread %.0f), max rel err %.2e"
+            % (key, cpu_b["value"], cpu_b["cores"],
+               "%.0f" % cpu_b["march_native"]["value"] if "march_native" in cpu_b else "-",
+               cpu_b["single_thread"]["value"], cpu_b["max_rel_err_vs_cpu"]))
+    n_near = float(r["n_near"].mean()) if r is not None else None
+    n_far = float(r["n_far"].mean()) if r is not None else None
+    kern_s = sum(ker_ms) / len(ker_ms) / 1e3
+    b_pose = pose_bytes(cx.lig.n, n_far or 0.0, n_near or 0.0, desolv, cx.rec.n) if r is not None else None
+    achieved = b_pose * N / kern_s / 1e9 if b_pose else None
+    tkey = {"workload": workload, "family": method, "poses_per_launch": N,
+            "tiles": [tiles["cell"], tiles["cell_x"], tiles["block"], tiles["chunk"]]}
+    traffic = measured_traffic(tkey)
+    ctx.close()
+    return {
+        "value": value, "unit": "pose-evals/s", "poses_per_step": N, "steps": K, "warmup": W,
+        "ms_per_step": wall / K * 1e3, "dtype": "fp32",
+        "steady_state": {"pose_evals_per_s": steady, "calls": reps} if steady else None,
+        "resident_pose_evals_per_s": resident, "resident_equals_host_calls": same,
+        "tiles": tiles, "flags": flags,
+        "near_field": ({"neighbour_lists": {"spacing_A": 1.0, "entries": n_list, "nodes": list(list_dims)}}
+                       if n_list else "cell-row walk"),
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "traffic_source": TRAFFIC_FILE if traffic else None,
+                     "kernel": "score_kernel", "kernel_ms": kern_s * 1e3,
+                     "prepass_ms": sum(pre_ms) / len(pre_ms), "timed_launches": len(ker_ms),
+                     "bytes_per_pose": b_pose,
+                     "n_near_per_pose": n_near, "n_far_per_pose": n_far},
+        "cpu_baseline": cpu_b,
+        "max_rel_err_vs_oracle": cpu_b["max_rel_err_vs_cpu"] if cpu_b else None,
+    }
+
+
+def bench_c4(K, W, args, cpu):
+    """64 complexes (receptors 2k-20k atoms, ligands 500-5k, 100-500
+    glowworms, dna/pydock/vdw mix): one batch launch per GSO step.  The value
+    is the host-buffer step (poses H2D, kernel, scores D2H, replayed from a
+    captured graph); the resident batch and 64 sequential per-complex launches
+    are reported beside it."""
+    import numpy as np
+    import torch
+    import lklight
+    from lklight import synth
     from oracle import lkl_oracle as O
-    orc = O.Oracle(cx)
-    n0 = max(threads * 2, 32)
+
+    dev = torch.device("cuda:0")
     t = time.perf_counter()
-    r = orc.batch(method, poses[:n0], nthreads=threads, counts=True)
-    dt = time.perf_counter() - t
-    n = int(min(poses.shape[0], max(n0, n0 * budget_s / max(dt, 1e-3))))
-    t = time.perf_counter()
-    r = orc.batch(method, poses[:n], nthreads=threads, counts=True)
-    dt = time.perf_counter() - t
-    return n / dt, n, dt, r
+    cxs = synth.make_config4(64)
+    ctxs, flags, counts = [], [], []
+    for cx in cxs:
+        fam = cx.meta["family"]
+        ctxs.append(lklight.Context(cx, with_field=False, device_field=(fam != "vdw"), cell_size=4.0))
+        flags.append(lklight.family_flags(fam))
+        counts.append(cx.meta["glowworms"])
+    log("c4: 64 complexes + device phi in %.1f s" % (time.perf_counter() - t))
+    P = sum(counts)
+    offs = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    step_poses = np.stack([np.concatenate([synth.make_poses(n, cx.meta["rec_axes"], 1000 * s_ + k)
+                                           for k, (n, cx) in enumerate(zip(counts, cxs))])
+                           for s_ in range(K + W)])
+    m = lklight.Multi(ctxs, flags, counts)
+    if args.tune:
+        tb, tm, _ = m.autotune(step_poses[0])
+        tiles = {"block": tb, "chunk_mult": tm, "source": "autotuned now"}
+    else:
+        tiles = dict(lklight.tuned_tiles("c4"))
+        m.set_tiles(tiles["block"], tiles["chunk_mult"])
+        tiles["source"] = "lklight-gpu_amd/lklight/tiles.json"
+    # the metric: host-buffer steps replayed from the captured graph
+    m.use_graph(True)
+    for i in range(W):
+        m.score(step_poses[i])
+    outs = []
+    t0 = time.perf_counter()
+    for k in range(K):
+        outs.append(m.score(step_poses[W + k]))
+    wall = time.perf_counter() - t0
+    value = P * K / wall
+    # resident batch (kernel events) and 64 sequential launches
+    d_poses = torch.from_numpy(step_poses).to(dev)
+    d_out = torch.zeros((K + W, P, 2), dtype=torch.float64, device=dev)
+    d_seq = torch.zeros((K + W, P, 2), dtype=torch.float64, device=dev)
+    stream = torch.cuda.Stream()
+    sh = stream.cuda_stream
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(K)]
+
+    def run_batch(i, k=None):
+        if k is not None:
+            ev[k][0].record(stream)
+        m.score_async(d_poses[i].data_ptr(), d_out[i].data_ptr(), sh)
+        if k is not None:
+            ev[k][1].record(stream)
+
+    def run_seq(i, k=None):
+        for j, ctx in enumerate(ctxs):
+            ctx.score_async(d_poses[i, offs[j]:offs[j + 1]].data_ptr(), counts[j], flags[j],
+                            d_seq[i, offs[j]:offs[j + 1]].data_ptr(), None, sh)
+
+    res = {}
+    for name, fn in (("batch", run_batch), ("sequential", run_seq)):
+        for i in range(W):
+            fn(i)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(K):
+            fn(W + k, k)
+        torch.cuda.synchronize()
+        res[name] = time.perf_counter() - t0
+    kern_ms = sum(a.elapsed_time(b) for a, b in ev) / K
+    same_seq = bool(torch.equal(d_out[W:], d_seq[W:]))
+    same_host = bool(np.array_equal(d_out[W:].cpu().numpy(), np.stack(outs)))
+    log("c4: host-buffer graph steps %.0f pose-evals/s, resident batch %.0f, sequential %.0f, kernel %.3f ms"
+        % (value, P * K / res["batch"], P * K / res["sequential"], kern_ms))
+    # oracle sample: 16 poses of every complex from the first timed step
+    # (host phi built for the sampled complex, then dropped)
+    cpu_b, max_rel, b_launch = None, None, None
+    if cpu.enabled:
+        th = cpu.info["threads"]
+        g = outs[0]
+        bytes_est, t_cpu, n_cpu, max_rel = 0.0, 0.0, 0, 0.0
+        for j, cx in enumerate(cxs):
+            fam = cx.meta["family"]
+            if fam != "vdw":
+                cx.build_field()
+            orc = O.Oracle(cx)
+            ps = step_poses[W, offs[j]:offs[j] + 16]
+            t = time.perf_counter()
+            r = orc.batch(fam, ps, nthreads=th, counts=True)
+            t_cpu += time.perf_counter() - t
+            n_cpu += ps.shape[0]
+            gj = g[offs[j]:offs[j] + 16]
+            max_rel = max(max_rel, rel_err(fam, gj[:, 0], gj[:, 1], None, r))
+            bp = pose_bytes(cx.lig.n, float(r["n_far"].mean()), float(r["n_near"].mean()))
+            bytes_est += bp * counts[j]
+            cx.phi = None
+            del orc
+        b_launch = bytes_est
+        cpu_b = {"value": n_cpu / t_cpu, "unit": "pose-evals/s", "cores": th, "kind": "port",
+                 "sample": "16 poses of each of the 64 complexes (%d), one oracle batch per complex, %s host, "
+                           "-O3 -march=x86-64" % (n_cpu, cpu.info["model"]),
+                 "gpu_over_cpu": value / (n_cpu / t_cpu), "max_rel_err_vs_cpu": max_rel}
+        log("c4: cpu %.0f pose-evals/s, max rel err %.2e" % (cpu_b["value"], max_rel))
+    achieved = b_launch / (kern_ms / 1e3) / 1e9 if b_launch else None
+    fams = {str(f): int(n) for f, n in zip(*np.unique([c.meta["family"] for c in cxs], return_counts=True))}
+    m.close()
+    for c in ctxs:
+        c.close()
+    return {
+        "value": value, "unit": "pose-evals/s", "poses_per_step": P, "steps": K, "warmup": W,
+        "ms_per_step": wall / K * 1e3, "dtype": "fp32",
+        "workload": "c4: 64 complexes, receptors 2k-20k atoms, ligands 500-5k, %d poses/step" % P,
+        "families": fams, "tiles": tiles,
+        "resident_pose_evals_per_s": P * K / res["batch"],
+        "sequential_pose_evals_per_s": P * K / res["sequential"],
+        "batch_over_sequential": res["sequential"] / res["batch"],
+        "bitwise_equal_to_sequential": same_seq, "host_steps_equal_resident": same_host,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "kernel": "multi_kernel", "kernel_ms": kern_ms, "bytes_per_launch": b_launch},
+        "cpu_baseline": cpu_b, "max_rel_err_vs_oracle": max_rel,
+    }
+
+
+def retune(args):
+    """Run the autotuners on every workload and print a fresh tiles table."""
+    import lklight
+    from lklight import synth
+    out = {}
+    cx2 = synth.make_config("c2")
+    cx2.build_field()
+    cx3 = synth.make_config("c3")
+    cx3.build_field()
+    for key, cx in (("c2-dna", cx2), ("c2-pydock", cx2), ("c2-vdw", cx2), ("c3-cpydock", cx3)):
+        workload, method = key.split("-")
+        desolv = method == "cpydock"
+        ctx = lklight.Context(cx, desolv=desolv, with_field=method != "vdw")
+        flags = lklight.family_flags(method) | (lklight.F_DESOLV if desolv else 0)
+        N = 400 if workload == "c3" else 1200
+        cell, cell_x, block, chunk, table = ctx.autotune(synth.make_poses(N, cx.meta["rec_axes"], 2024), flags)
+        out[key] = {"cell": cell, "cell_x": cell_x, "block": block, "chunk": chunk,
+                    "best_ms": min(r[4] for r in table)}
+        log("%s: %s" % (key, out[key]))
+        ctx.close()
+    print(json.dumps(out))
 
 
 def main():
@@ -77,25 +451,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--family", default="dna", choices=["dna", "pydock", "vdw", "cpydock"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cell", type=float, default=0.0)
-    ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--chunk", type=int, default=-1)
-    ap.add_argument("--cell-x", type=float, default=0.0, help="receptor cell edge along x (0 = cubic)")
-    ap.add_argument("--list-h", type=float, default=-1.0,
-                    help="neighbour-list node spacing in A (0 = cell-row walk; default: the library's 1 A)")
-    ap.add_argument("--no-tune", action="store_true", help="skip the tile autotuner")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
-                    help="c2: 12.6k-atom metric workload; c3: 8k x 3k cpydock, 400 poses/step; "
-                         "c4: 64-complex batch; c5: surrogate-screened GSO vs plain exact GSO")
-    ap.add_argument("--no-surrogate", action="store_true",
-                    help="c2: skip the surrogate-screened GSO comparison")
+    ap.add_argument("--only", default="",
+                    help="comma list of %s (default: all; the first listed is the headline)"
+                         % ", ".join(ALL_CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="seconds of oracle work for the headline CPU baseline (others get a third)")
+    ap.add_argument("--steady", type=int, default=200,
+                    help="host-buffer calls after the timed steps, for a steady-state rate (0: skip)")
+    ap.add_argument("--tune", action="store_true", help="run the autotuner instead of the stored tiles")
+    ap.add_argument("--retune", action="store_true", help="only run the autotuners and print the tiles")
+    ap.add_argument("--workload", default="", choices=["", "c5"],
+                    help="c5: only the surrogate-screened GSO against plain exact GSO")
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate-screened GSO comparison")
     ap.add_argument("--no-gso", action="store_true",
-                    help="c2: skip the host-driven vs device-resident GSO step comparison")
+                    help="skip the host-driven vs device-resident GSO step comparison")
     ap.add_argument("--gso-only", action="store_true",
-                    help="c2: only the host-driven vs device-resident GSO step comparison (one JSON line)")
+                    help="only the host-driven vs device-resident GSO step comparison (one JSON line)")
     ap.add_argument("--sur-train", type=int, default=2_000_000, help="c5: training poses")
     ap.add_argument("--sur-epochs", type=int, default=6)
     ap.add_argument("--sur-swarms", type=int, default=40)
@@ -124,7 +496,6 @@ def main():
                           "built)" % args.gpus}))
         sys.exit(2)
 
-    import numpy as np
     import torch
     import lklight
     from lklight import synth
@@ -132,160 +503,83 @@ def main():
     if not lklight.device_available():
         raise SystemExit("no gfx950 device: the benchmark measures the HIP path only")
     torch.cuda.init()
-    dev = torch.device("cuda:0")
-    if args.workload == "c4":
-        return bench_c4(args)
+    if args.retune:
+        return retune(args)
     if args.workload == "c5":
         print(json.dumps(surrogate_experiment(args)))
         return
-
-    if args.workload == "c3":
-        args.family = "cpydock"
-        log("generating 8,000-atom receptor / 3,000-atom ligand (radii 1.5-2.0 A)")
-        cx = synth.make_config("c3")
-    else:
-        log("generating 12,625-atom receptor / 1,500-atom ligand")
-        cx = synth.make_config("c2")
-    t = time.perf_counter()
-    cx.build_field()
-    log("phi %s built on GPU in %.2f s" % (cx.dims, time.perf_counter() - t))
-    desolv = args.family == "cpydock"
-    ctx = lklight.Context(cx, desolv=desolv)
-    flags = lklight.family_flags(args.family) | (lklight.F_DESOLV if desolv else 0)
-
-    K, W, N = args.steps, args.warmup, (400 if args.workload == "c3" else POSES_PER_STEP)
-    poses = synth.make_poses(N * (K + W), cx.meta["rec_axes"], 2024)
-    if args.list_h >= 0:
-        ctx.set_list_spacing(args.list_h)
-    tune = None
-    if args.cell or args.block or args.chunk >= 0 or args.cell_x:
-        tiles = (args.cell or 5.0, args.block or 256, max(args.chunk, 0), args.cell_x or (args.cell or 5.0))
-        ctx.set_tiles(tiles[0], tiles[1], tiles[2], cell_x=args.cell_x)
-    elif not args.no_tune:
-        t = time.perf_counter()
-        cell, cell_x, block, chunk, table = ctx.autotune(poses[:N], flags)
-        tiles = (cell, block, chunk, cell_x)
-        tune = {"table": [[round(a, 2), round(b, 2), int(c), int(d), round(e, 4)] for a, b, c, d, e in table],
-                "columns": ["cell_A", "cell_x_A", "block", "chunk", "ms"],
-                "seconds": round(time.perf_counter() - t, 2)}
-        log("autotuner: cell %.2f A (x %.2f A), block %d, chunk %d (%.1f s)"
-            % (cell, cell_x, block, chunk, tune["seconds"]))
-    else:
-        tiles = (5.0, 256, 0, 5.0)
-    n_list, list_dims = ctx.list_info()
-    log("near field: %s" % ("neighbour lists, %d entries over %s nodes" % (n_list, list_dims) if n_list
-                            else "cell-row walk"))
     if args.gso_only:
-        res = gso_step_compare(cx, ctx, args.family)
-        print(json.dumps({"metric": "GSO step time, %s, 6 swarms x 200 glowworms (device-resident)" % args.family,
+        cx = synth.make_config("c2")
+        cx.build_field()
+        ctx = lklight.Context(cx)
+        t = lklight.tuned_tiles("c2-dna")
+        ctx.set_tiles(t["cell"], t["block"], t["chunk"], cell_x=t["cell_x"])
+        res = gso_step_compare(cx, ctx, "dna")
+        print(json.dumps({"metric": "GSO step time, dna, 6 swarms x 200 glowworms (device-resident)",
                           "value": res["device"]["ms_per_step"], "unit": "ms/step", "higher_is_better": False,
                           "n_gpus": 1, "gso_step": res}))
         return
-    d_poses = torch.from_numpy(poses).to(dev)
-    d_out = torch.zeros((K + W, N, 2), dtype=torch.float64, device=dev)
-    d_s = torch.zeros((K + W, N), dtype=torch.float64, device=dev)
-    # a dedicated (default-priority) stream: its handle is non-null, so the
-    # kernels and the timing events share it
-    stream = torch.cuda.Stream()
-    sh = stream.cuda_stream
-    assert sh, "expected a non-null stream handle"
+    K, W = args.steps, args.warmup
+    keys = [k for k in (args.only.split(",") if args.only else ALL_CONFIGS) if k]
+    for k in keys:
+        if k not in ALL_CONFIGS:
+            raise SystemExit("unknown workload %r (choose from %s)" % (k, ", ".join(ALL_CONFIGS)))
+    info = host_info()
+    cpu = CpuLeg(info, enabled=not args.no_cpu)
+    if cpu.enabled:
+        log("cpu legs: %d threads (nproc %d, affinity %d, cgroup quota %s), %s"
+            % (info["threads"], info["nproc"], info["affinity_cpus"], info["cgroup_cpu_quota"], info["model"]))
+    cache = {}
 
-    def step(i):
-        ctx.score_async(d_poses[i * N:(i + 1) * N].data_ptr(), N, flags, d_out[i].data_ptr(),
-                        d_s[i].data_ptr() if desolv else None, sh)
+    def complex_of(workload):
+        if workload not in cache:
+            log("generating the %s complex" % workload)
+            cx = synth.make_config(workload)
+            t = time.perf_counter()
+            cx.build_field()
+            log("phi %s built on GPU in %.2f s" % (cx.dims, time.perf_counter() - t))
+            cache[workload] = cx
+        return cache[workload]
 
-    torch.cuda.synchronize()
-    for i in range(W):
-        step(i)
-    torch.cuda.synchronize()
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-          for _ in range(K)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(K):
-        ev[k][0].record(stream)
-        step(W + k)
-        ev[k][1].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
-    avg_kernel_s = sum(kern_ms) / len(kern_ms) / 1e3
-    value = N * K / wall
-    log("timed %d steps: %.3f s wall, kernel avg %.3f ms" % (K, wall, avg_kernel_s * 1e3))
-
-    # The same steps through the blocking host-buffer call (poses H2D, kernel,
-    # scores D2H, synchronise): the PCIe-inclusive rate, reported beside
-    # `value` (which has the poses resident in HBM), never as it.
-    host_poses = [np.ascontiguousarray(poses[(W + k) * N:(W + k + 1) * N]) for k in range(min(K, 20))]
-    ctx.score(host_poses[0], flags)
-    t0 = time.perf_counter()
-    for hp in host_poses:
-        ctx.score(hp, flags)
-    host_io_rate = N * len(host_poses) / (time.perf_counter() - t0)
-    log("host buffers (H2D + kernel + D2H per call): %.0f pose-evals/s" % host_io_rate)
-
-    # CPU baseline (oracle) on a bounded sample of the same poses; its pair
-    # counts also give the algorithmic bytes of the roofline figure.
-    threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-    cpu = None
-    n_near = n_far = None
-    sample_poses = poses[W * N:]
-    if not args.no_cpu:
-        log("cpu baseline: oracle on %d threads" % threads)
-        rate, n, dt, r = cpu_baseline(cx, sample_poses, args.family, threads)
-        n_near, n_far = float(r["n_near"].mean()), float(r["n_far"].mean())
-        g = d_out[W:].reshape(-1, 2)[:n].cpu().numpy()
-        sc_g = lklight.fold_scores(args.family, g[:, 0], g[:, 1],
-                                   d_s[W:].reshape(-1)[:n].cpu().numpy() if desolv else None)
-        sc_o = lklight.fold_scores(args.family, r["e"], r["v"], r["s"] if desolv else None)
-        max_rel = float(np.max(np.abs(sc_g - sc_o) / np.maximum(np.abs(sc_o), 1.0)))
-        model = "unknown"
-        try:
-            for ln in open("/proc/cpuinfo"):
-                if ln.startswith("model name"):
-                    model = ln.split(":", 1)[1].strip()
-                    break
-        except OSError:
-            pass
-        cpu = {"value": rate, "unit": "pose-evals/s", "cores": threads, "kind": "port",
-               "sample": "first %d poses of the timed workload (%.1f s), %s, f64 C restatement "
-                         "of the reference CPU grid path, -O3 -march=x86-64" % (n, dt, model),
-               "gpu_over_cpu": value / rate, "max_rel_err_vs_cpu": max_rel}
-    b_pose = pose_bytes(cx.lig.n, n_far or 0.0, n_near or 0.0, desolv, cx.rec.n)
-    achieved = b_pose * N / avg_kernel_s / 1e9 if n_near is not None else None
+    results = {}
+    for i, key in enumerate(keys):
+        budget = args.cpu_budget if i == 0 else args.cpu_budget / 3
+        if key == "c4":
+            results[key] = bench_c4(K, W, args, cpu)
+        else:
+            results[key] = bench_single(key, complex_of(key.split("-")[0]), K, W, args, cpu, budget)
+    head_key = keys[0]
+    head = results[head_key]
+    if head_key == "c2-dna":
+        metric = METRIC
+        workload = ("c2-dna: 12,625-atom receptor x 1,500-atom ligand, 1200 poses/step (6 swarms x 200 "
+                    "glowworms), phi %s at 0.5 A, blocking host-buffer scoring call per step"
+                    % "x".join(map(str, cache["c2"].dims)))
+    else:
+        metric = "pose-evals/s (%s)" % head_key
+        workload = head.get("workload", head_key)
     out = {
-        "metric": METRIC if (args.workload, args.family) == ("c2", "dna")
-        else "pose-evals/s (%s, %s)" % (args.family, "8k-atom rec" if args.workload == "c3" else "12.6k-atom rec"),
-        "value": value, "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
-        "warmup": W, "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "metric": metric, "value": head["value"], "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": ("c2-%s: 12,625-atom receptor x 1,500-atom ligand, %d poses/step "
-                                "(6 swarms x 200 glowworms), phi %s at 0.5 A"
-                                % (args.family, N, "x".join(map(str, cx.dims))))
-                   if args.workload == "c2" else
-                   ("c3-cpydock: 8,000-atom receptor x 3,000-atom ligand (radii 1.5-2.0 A), %d poses/step, "
-                    "contact-SASA desolvation over Lee & Richards areas (probe 1.4 A), phi %s at 0.5 A"
-                    % (N, "x".join(map(str, cx.dims)))),
-                   "family": args.family, "flags": flags, "poses_per_step": N,
-                   "cell_edge_A": tiles[0], "cell_edge_x_A": tiles[3], "block": tiles[1],
-                   "ligand_atoms_per_tile": tiles[2],
-                   "near_field": ({"neighbour_lists": {"spacing_A": args.list_h if args.list_h > 0 else 1.0,
-                                                       "entries": n_list, "nodes": list(list_dims)}}
-                                  if n_list else "cell-row walk"),
-                   "autotune": tune},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None,
-                     "traffic": measured_traffic(args.workload, args.family, N),
-                     "traffic_source": TRAFFIC_FILE if measured_traffic(args.workload, args.family, N) else None,
-                     "kernel_ms": avg_kernel_s * 1e3, "bytes_per_pose": b_pose,
-                     "n_near_per_pose": n_near, "n_far_per_pose": n_far},
-        "cpu_baseline": cpu,
-        "host_io_pose_evals_per_s": host_io_rate,
+        "config": {"workload": workload, "poses_per_step": head["poses_per_step"],
+                   "tiles": head.get("tiles"), "near_field": head.get("near_field")},
+        "roofline": head["roofline"], "cpu_baseline": head["cpu_baseline"],
+        "max_rel_err_vs_oracle": head["max_rel_err_vs_oracle"],
+        "resident_pose_evals_per_s": head["resident_pose_evals_per_s"],
+        "steady_state": head.get("steady_state"),
+        "configs": {k: v for k, v in results.items() if k != head_key},
     }
-    if args.workload == "c2" and not args.no_gso:
-        out["gso_step"] = gso_step_compare(cx, ctx, args.family)
-        out["restraints"] = interface_timing(cx, ctx, d_poses[W * N:(W + 1) * N], N, stream)
-    ctx.close()
+    if "c2" in cache and not args.no_gso:
+        cx = cache["c2"]
+        ctx = lklight.Context(cx)
+        t = lklight.tuned_tiles("c2-dna")
+        ctx.set_tiles(t["cell"], t["block"], t["chunk"], cell_x=t["cell_x"])
+        out["gso_step"] = gso_step_compare(cx, ctx, "dna")
+        stream = torch.cuda.Stream()
+        d_poses = torch.from_numpy(synth.make_poses(1200, cx.meta["rec_axes"], 2024)).cuda()
+        out["restraints"] = interface_timing(cx, ctx, d_poses, 1200, stream)
+        ctx.close()
     if not args.no_surrogate:
         out["surrogate"] = surrogate_experiment(args)
     print(json.dumps(out))
@@ -466,87 +760,6 @@ def surrogate_experiment(args):
     ctx.close()
     return out
 
-
-def bench_c4(args):
-    """64 complexes (receptors 2k-20k atoms, ligands 500-5k, 100-500 glowworms,
-    dna/pydock/vdw mix): one batch launch per GSO step vs 64
-    sequential per-complex launches, both on device-resident poses."""
-    import numpy as np
-    import torch
-    import lklight
-    from lklight import synth
-
-    dev = torch.device("cuda:0")
-    t = time.perf_counter()
-    cxs = synth.make_config4(64)
-    log("64 complexes generated in %.1f s" % (time.perf_counter() - t))
-    t = time.perf_counter()
-    ctxs, flags, counts = [], [], []
-    for cx in cxs:
-        fam = cx.meta["family"]
-        ctxs.append(lklight.Context(cx, with_field=False, device_field=(fam != "vdw"), cell_size=4.0))
-        flags.append(lklight.family_flags(fam))
-        counts.append(cx.meta["glowworms"])
-    log("contexts + device phi in %.1f s" % (time.perf_counter() - t))
-    K, W = args.steps, args.warmup
-    P = sum(counts)
-    step_poses = []
-    for s_ in range(K + W):
-        step_poses.append(np.concatenate([synth.make_poses(n, cx.meta["rec_axes"], 1000 * s_ + k)
-                                          for k, (n, cx) in enumerate(zip(counts, cxs))]))
-    m = lklight.Multi(ctxs, flags, counts)
-    t = time.perf_counter()
-    tb, tm, table = m.autotune(step_poses[0])
-    log("batch autotune: block %d, chunk_mult %d (%.1f s)" % (tb, tm, time.perf_counter() - t))
-    d_poses = torch.from_numpy(np.stack(step_poses)).to(dev)
-    d_out = torch.zeros((K + W, P, 2), dtype=torch.float64, device=dev)
-    d_seq = torch.zeros((K + W, P, 2), dtype=torch.float64, device=dev)
-    stream = torch.cuda.Stream()
-    sh = stream.cuda_stream
-    offs = np.concatenate([[0], np.cumsum(counts)])
-
-    def run_batch(i):
-        m.score_async(d_poses[i].data_ptr(), d_out[i].data_ptr(), sh)
-
-    def run_seq(i):
-        for k, ctx in enumerate(ctxs):
-            ctx.score_async(d_poses[i, offs[k]:offs[k + 1]].data_ptr(), counts[k], flags[k],
-                            d_seq[i, offs[k]:offs[k + 1]].data_ptr(), None, sh)
-
-    res = {}
-    for name, fn in (("batch", run_batch), ("sequential", run_seq)):
-        torch.cuda.synchronize()
-        for i in range(W):
-            fn(i)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for k in range(K):
-            fn(W + k)
-        torch.cuda.synchronize()
-        res[name] = time.perf_counter() - t0
-    same = bool(torch.equal(d_out[W:], d_seq[W:]))
-    # host-driven steps replayed from the captured graph (H2D + kernel + D2H)
-    m.use_graph(True)
-    host_p = np.stack(step_poses)
-    m.score(host_p[0])
-    t0 = time.perf_counter()
-    for k in range(min(K, 20)):
-        m.score(host_p[W + k])
-    graph_s = (time.perf_counter() - t0) / min(K, 20)
-    fams = {str(f): int(n) for f, n in zip(*np.unique([c.meta["family"] for c in cxs],
-                                                      return_counts=True))}
-    out = {"metric": "aggregate pose-evals/s, 64-complex batch (one launch per step)",
-           "value": P * K / res["batch"], "unit": "pose-evals/s", "n_gpus": 1, "steps": K,
-           "warmup": W, "ms_per_step": res["batch"] / K * 1e3, "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
-           "config": {"workload": "c4: 64 complexes, receptors 2k-20k atoms, ligands 500-5k, "
-                                  "%d poses/step" % P, "families": fams,
-                      "block": tb, "chunk_mult": tm},
-           "sequential_pose_evals_per_s": P * K / res["sequential"],
-           "batch_over_sequential": res["sequential"] / res["batch"],
-           "bitwise_equal_to_sequential": same,
-           "graph_step_ms_host_io": graph_s * 1e3}
-    print(json.dumps(out, default=str))
 
 
 if __name__ == "__main__":

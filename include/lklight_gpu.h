@@ -159,6 +159,14 @@ int lkl_ctx_set_pose_order(lkl_ctx* ctx, int enable);
 /* Current setting: 1 / 0, or -1 for a NULL context. */
 int lkl_ctx_pose_order(const lkl_ctx* ctx);
 
+/* Kernel timing (no reference counterpart; measurement only).  With timing on,
+ * every launch of the context (blocking or asynchronous) records HIP events on
+ * its stream around the dispatch-order pre-pass and around the scoring
+ * kernel.  lkl_ctx_kernel_ms waits for the last launch and returns both
+ * durations in ms (prepass_ms is 0 when that launch ran no pre-pass).  0 / -1. */
+int lkl_ctx_set_timing(lkl_ctx* ctx, int on);
+int lkl_ctx_kernel_ms(lkl_ctx* ctx, double* prepass_ms, double* score_ms);
+
 /* Scores ligands already placed: coords N*nl*3 f32 (pose-major, the
  * descriptor's ligand order), one pose per coordinate set, the context's
  * charges and parameters.  This is the input form of the reference's

@@ -74,6 +74,10 @@ struct DevComplex {
     // nodes + 1 offsets into nidx.
     const unsigned* noff;
     const unsigned short* nidx;
+    // the same lists with 32-bit indices (receptors of more than 65,535
+    // atoms, whose dummy index nr does not fit 16 bits); when set, nidx is
+    // NULL and the walks read these instead
+    const unsigned* nidx32;
     // the sorted receptor records interleaved for the list walk: rrec[2p] =
     // rxyzq[p], rrec[2p + 1] = rpar[p] (one 32-byte record per atom, nr + 1)
     const float4* rrec;

@@ -206,6 +206,8 @@ struct Swarm {
 
 typedef int (*lkl_score_fallback)(void* user, const double* poses, int n, double* scores);
 
+void lkl_set_error_text(const char* msg);  // lkl_score.hip: the thread's lkl_last_error()
+
 // Device-resident steps (lkl_gsod.hip).
 struct lkl_gsod_buf;
 void lkl_gsod_free(lkl_gsod_buf* d);
@@ -256,6 +258,14 @@ static double restraint_fold(double score, const int* cnt, const int* sizes) {
     return score;
 }
 
+// Record a failure of the driver: kept in G->err and reported through
+// lkl_last_error() like every other entry point's.
+static int gso_fail(lkl_gso* G, const std::string& msg) {
+    G->err = msg;
+    lkl_set_error_text(msg.c_str());
+    return -1;
+}
+
 // Score `poses` (n x 7) into `scores` (GPU first, then the fallback if any).
 static int score_batch(lkl_gso* G, const std::vector<double>& poses, int n, std::vector<double>& scores) {
     scores.assign(n, 0.0);
@@ -277,11 +287,18 @@ static int score_batch(lkl_gso* G, const std::vector<double>& poses, int n, std:
             G->scored += n;
             return 0;
         }
-        if (!G->fallback) return -1;
+        // dna.rs:1081-1085: a failed GPU call is re-scored on the CPU grid
+        // path when a fallback is registered; otherwise the GPU's message is
+        // the run's error
+        if (!G->fallback)
+            return gso_fail(G, std::string("GPU scoring failed and no fallback is registered: ") +
+                                   lkl_last_error());
     }
-    if (!G->fallback) { G->err = "no scorer: no GPU context and no fallback"; return -1; }
+    if (!G->fallback) return gso_fail(G, "no scorer: no GPU context and no fallback");
     ++G->fallback_calls;
-    return G->fallback(G->fallback_user, poses.data(), n, scores.data()) == 0 ? 0 : -1;
+    if (G->fallback(G->fallback_user, poses.data(), n, scores.data()) != 0)
+        return gso_fail(G, "the registered CPU fallback scorer failed");
+    return 0;
 }
 
 // The pose set of dna.rs:497-519 (DNA::dump_metal_dataset): per pose, three

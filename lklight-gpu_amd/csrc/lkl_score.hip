@@ -226,17 +226,24 @@ __device__ __forceinline__ void near_list(const DevComplex& C, double x64, doubl
     const float xl = (float)(x64 - (double)xh);
     const float yl = (float)(y64 - (double)yh);
     const float zl = (float)(z64 - (double)zh);
-    const unsigned short dn = (unsigned short)C.nr;  // the dummy record
+    const unsigned dn = (unsigned)C.nr;  // the dummy record
     for (unsigned p0 = b; p0 < en; p0 += 4u * U) {
         int id[4 * U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            ushort4 q = make_ushort4(dn, dn, dn, dn);
-            if (u == 0 || p0 + 4u * u < en) q = *reinterpret_cast<const ushort4*>(C.nidx + p0 + 4u * u);
-            id[4 * u] = q.x;
-            id[4 * u + 1] = q.y;
-            id[4 * u + 2] = q.z;
-            id[4 * u + 3] = q.w;
+            uint4 q = make_uint4(dn, dn, dn, dn);
+            if (u == 0 || p0 + 4u * u < en) {
+                if (C.nidx32) {
+                    q = *reinterpret_cast<const uint4*>(C.nidx32 + p0 + 4u * u);
+                } else {
+                    const ushort4 h = *reinterpret_cast<const ushort4*>(C.nidx + p0 + 4u * u);
+                    q = make_uint4(h.x, h.y, h.z, h.w);
+                }
+            }
+            id[4 * u] = (int)q.x;
+            id[4 * u + 1] = (int)q.y;
+            id[4 * u + 2] = (int)q.z;
+            id[4 * u + 3] = (int)q.w;
         }
         float4 rq[4 * U], rp[4 * U];
 #pragma unroll
@@ -320,7 +327,7 @@ template <bool FILL>
 __global__ void __launch_bounds__(NLT)
 nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz, float h, int nx, int ny,
              int nz, float rad, unsigned pad, int lanemajor, unsigned* __restrict__ cnt_off,
-             unsigned short* __restrict__ idx) {
+             unsigned short* __restrict__ idx, unsigned* __restrict__ idx32) {
 #pragma clang fp contract(off)
     __shared__ float4 buf[NLCAP];
     __shared__ int wave_cnt[NLT / 64];
@@ -378,8 +385,11 @@ nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float o
                     const float4 b = buf[q];
                     const float dx = px - b.x, dy = py - b.y, dz = pz - b.z;
                     if (dx * dx + dy * dy + dz * dz <= rad2) {
-                        if (FILL && base_out + k < end_out)
-                            idx[base_out + (lanemajor ? lane_major(k) : k)] = (unsigned short)__float_as_int(b.w);
+                        if (FILL && base_out + k < end_out) {
+                            const unsigned at = base_out + (lanemajor ? lane_major(k) : k);
+                            if (idx32) idx32[at] = (unsigned)__float_as_int(b.w);
+                            else idx[at] = (unsigned short)__float_as_int(b.w);
+                        }
                         ++k;
                     }
                 }
@@ -392,7 +402,11 @@ nlist_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float o
     if (!valid) return;
     const unsigned padded = (k + pad - 1u) / pad * pad;
     if (FILL) {
-        for (; base_out + k < end_out; ++k) idx[base_out + (lanemajor ? lane_major(k) : k)] = (unsigned short)nr;
+        for (; base_out + k < end_out; ++k) {
+            const unsigned at = base_out + (lanemajor ? lane_major(k) : k);
+            if (idx32) idx32[at] = (unsigned)nr;
+            else idx[at] = (unsigned short)nr;
+        }
     } else {
         cnt_off[node] = padded;
     }
@@ -510,7 +524,7 @@ constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 constexpr int GK = 8;        // entries per lane per step
 constexpr int GB = 4;        // step-count buckets of the active atoms
 
-template <bool FAR, bool ELEC, bool CLASH, int DES, bool TYPED = false>
+template <bool FAR, bool ELEC, bool CLASH, int DES, bool TYPED = false, bool WIDE = false>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                             unsigned* __restrict__ minR, long long& e, long long& v,
                                             long long& s) {
@@ -643,7 +657,15 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         // waits for its records only
         static_assert(GL * GK == LIST_PAD && GK == 8, "one lane-major block per step");
         auto step_ids = [&](unsigned p, int* out) {
-            const uint4 q = *reinterpret_cast<const uint4*>(C.nidx + (p < en ? p : C.ndum) + (unsigned)(r * GK));
+            const unsigned at = (p < en ? p : C.ndum) + (unsigned)(r * GK);
+            if constexpr (WIDE) {  // 32-bit lists: the lane's 8 entries are two 16-byte loads
+                const uint4 a = *reinterpret_cast<const uint4*>(C.nidx32 + at);
+                const uint4 c = *reinterpret_cast<const uint4*>(C.nidx32 + at + 4u);
+                out[0] = (int)a.x; out[1] = (int)a.y; out[2] = (int)a.z; out[3] = (int)a.w;
+                out[4] = (int)c.x; out[5] = (int)c.y; out[6] = (int)c.z; out[7] = (int)c.w;
+                return;
+            }
+            const uint4 q = *reinterpret_cast<const uint4*>(C.nidx + at);
             out[0] = (int)(q.x & 0xffffu);
             out[1] = (int)(q.x >> 16);
             out[2] = (int)(q.y & 0xffffu);
@@ -731,10 +753,13 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         // every launch over lists takes the grouped walk, desolvation included
         // (its receptor minima are LDS atomics, its ligand minima a shuffle
         // reduction over the atom's lanes)
-        if constexpr (LIST)
-            near_groups<FAR, ELEC, CLASH, DES, TYPED>(C, M, j0, j1, lists, minR, e, v, s);
-        else
+        // (the index width is chosen per launch, outside the walk's loop)
+        if constexpr (LIST) {
+            if (C.nidx32) near_groups<FAR, ELEC, CLASH, DES, TYPED, true>(C, M, j0, j1, lists, minR, e, v, s);
+            else near_groups<FAR, ELEC, CLASH, DES, TYPED, false>(C, M, j0, j1, lists, minR, e, v, s);
+        } else {
             near_sorted<LIST, FAR, ELEC, CLASH, DES>(C, M, j0, j1, lists, minR, e, v, s);
+        }
     } else if constexpr (DES == DES_NONE && !PERATOM) {
         {
             for (int j = j0 + tid; j < j1; j += bs) {
@@ -824,8 +849,13 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
     if (C.wgrid) {
         long long s_unused = 0;
         if constexpr (LIST) {
-            if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
-            else near_groups<FAR, ELEC, CLASH, DES_NONE, false>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+            if (C.nidx32) {
+                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else near_groups<FAR, ELEC, CLASH, DES_NONE, false, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+            } else {
+                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else near_groups<FAR, ELEC, CLASH, DES_NONE, false>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+            }
         } else
             near_sorted<false, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
         return;
@@ -932,6 +962,7 @@ constexpr float IFACE_HI = 15.211f, IFACE_LO = 15.209f;
 struct IfaceDev {
     const unsigned* noff;          // short lists over the main lists' nodes
     const unsigned short* nidx;
+    const unsigned* nidx32;        // the same with 32-bit indices (NULL unless nidx is)
     const int* r_off;
     const int* r_atoms;  // sorted receptor indices
     const int* l_off;
@@ -965,7 +996,7 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
         const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
         bool hit = false;
         for (unsigned p = b; p < en; ++p) {
-            const int id = (int)I.nidx[p];
+            const int id = I.nidx32 ? (int)I.nidx32[p] : (int)I.nidx[p];
             const float4 r = C.rrec[2 * id];
             const float dx = (xh - r.x) + xl, dy = (yh - r.y) + yl, dz = (zh - r.z) + zl;
             const float d2 = dx * dx + dy * dy + dz * dz;
@@ -1104,6 +1135,7 @@ struct lkl_ctx {
     float list_h = 1.0f;  // neighbour-list node spacing (A); 0 = cell-row walk
     unsigned* d_noff = nullptr;
     unsigned short* d_nidx = nullptr;
+    unsigned* d_nidx32 = nullptr;  // 32-bit indices instead (receptors above 65,535 atoms)
     int nn[3] = {0, 0, 0};
     float no[3] = {0.f, 0.f, 0.f};
     size_t list_entries = 0;  // 0 = no lists (row walk)
@@ -1145,6 +1177,7 @@ struct lkl_ctx {
     // lists, radius 3.9 A + half-diagonal + margin), built on first use
     unsigned* d_noff_if = nullptr;
     unsigned short* d_nidx_if = nullptr;
+    unsigned* d_nidx32_if = nullptr;
     unsigned iflist_version = ~0u;
     int ncx = 1, ncy = 1, ncz = 1;
     double cox = 0, coy = 0, coz = 0;
@@ -1174,6 +1207,15 @@ struct lkl_ctx {
     long long* d_acc = nullptr;  // per-pose tile accumulators (kept zero between launches)
     unsigned* d_cnt = nullptr;
     size_t cap_acc = 0;
+    // pinned host staging of the blocking call (poses in, scores out): one
+    // DMA each way and a single stream synchronisation per call
+    double* h_stage = nullptr;
+    size_t cap_stage = 0;
+    // optional kernel timing of every launch: events before the dispatch-order
+    // pre-pass, before the scoring kernel and after it, on the launch's stream
+    bool timing = false;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ev_prepass = false;  // the last timed launch ran the pre-pass
     unsigned version = 0;  // bumped whenever a device array is replaced
 };
 
@@ -1249,19 +1291,23 @@ static int build_phi8(lkl_ctx* c) {
 // skipped (row walk) when switched off, when an index would not fit 16 bits, or
 // when the node grid or the entry count exceeds the caps below.
 constexpr long long LIST_NODES_MAX = 1ll << 27;
-constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of indices
+constexpr long long LIST_ENTRIES_MAX = 1ll << 30;  // 2 GiB of 16-bit indices (4 GiB of 32-bit)
 
 // Count and fill one list set over the node grid (o, n, h) with radius rad,
 // each list padded to a multiple of pad (blocks lane-major if lanemajor);
 // outputs a fresh offsets array (nodes + 1) and index array.  Returns the
 // entry count, 0 when it exceeds LIST_ENTRIES_MAX (nothing allocated), or -1.
 static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float h, float rad, unsigned pad,
-                            bool lanemajor, unsigned** d_off, unsigned short** d_idx, unsigned* dummy_pos = nullptr) {
+                            bool lanemajor, unsigned** d_off, unsigned short** d_idx, unsigned** d_idx32,
+                            unsigned* dummy_pos = nullptr) {
+    // 16-bit indices while the dummy index nr fits them, else 32-bit
+    const bool wide = c->nr > 65535;
     const long long nodes = (long long)n[0] * n[1] * n[2];
     HIPCK("alloc", hipMalloc((void**)d_off, sizeof(unsigned) * (size_t)(nodes + 1)));
     dim3 grid((n[0] + NLX - 1) / NLX, (n[1] + NLY - 1) / NLY, (n[2] + NLZ - 1) / NLZ);
     hipLaunchKernelGGL(nlist_kernel<false>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, (unsigned short*)nullptr);
+                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, (unsigned short*)nullptr,
+                       (unsigned*)nullptr);
     HIPCK("launch", hipGetLastError());
     HIPCK("sync", hipStreamSynchronize(c->stream));
     std::vector<unsigned> off((size_t)nodes + 1);
@@ -1281,12 +1327,19 @@ static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float 
     off[nodes] = (unsigned)acc;
     HIPCK("upload", h2d(*d_off, off.data(), sizeof(unsigned) * off.size(), c->stream));
     // a block of pad dummy entries (index nr) after the last list
-    HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)(acc + pad)));
-    const std::vector<unsigned short> dum(pad, (unsigned short)c->nr);
-    HIPCK("upload", h2d(*d_idx + acc, dum.data(), sizeof(unsigned short) * pad, c->stream));
+    if (wide) {
+        HIPCK("alloc", hipMalloc((void**)d_idx32, sizeof(unsigned) * (size_t)(acc + pad)));
+        const std::vector<unsigned> dum(pad, (unsigned)c->nr);
+        HIPCK("upload", h2d(*d_idx32 + acc, dum.data(), sizeof(unsigned) * pad, c->stream));
+    } else {
+        HIPCK("alloc", hipMalloc((void**)d_idx, sizeof(unsigned short) * (size_t)(acc + pad)));
+        const std::vector<unsigned short> dum(pad, (unsigned short)c->nr);
+        HIPCK("upload", h2d(*d_idx + acc, dum.data(), sizeof(unsigned short) * pad, c->stream));
+    }
     if (dummy_pos) *dummy_pos = (unsigned)acc;
     hipLaunchKernelGGL(nlist_kernel<true>, grid, dim3(NLT), 0, c->stream, c->d_rxyzq, c->nr, o[0], o[1], o[2], h,
-                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, *d_idx);
+                       n[0], n[1], n[2], rad, pad, lanemajor ? 1 : 0, *d_off, wide ? nullptr : *d_idx,
+                       wide ? *d_idx32 : nullptr);
     HIPCK("launch", hipGetLastError());
     HIPCK("sync", hipStreamSynchronize(c->stream));
     return (long long)std::max(acc, 1ull);
@@ -1295,16 +1348,18 @@ static long long fill_lists(lkl_ctx* c, const float o[3], const int n[3], float 
 static void free_iface_lists(lkl_ctx* c) {
     if (c->d_noff_if) { (void)hipFree(c->d_noff_if); c->d_noff_if = nullptr; }
     if (c->d_nidx_if) { (void)hipFree(c->d_nidx_if); c->d_nidx_if = nullptr; }
+    if (c->d_nidx32_if) { (void)hipFree(c->d_nidx32_if); c->d_nidx32_if = nullptr; }
     c->iflist_version = ~0u;
 }
 
 static int build_lists(lkl_ctx* c) {
     if (c->d_noff) { (void)hipFree(c->d_noff); c->d_noff = nullptr; }
     if (c->d_nidx) { (void)hipFree(c->d_nidx); c->d_nidx = nullptr; }
+    if (c->d_nidx32) { (void)hipFree(c->d_nidx32); c->d_nidx32 = nullptr; }
     free_iface_lists(c);
     c->list_entries = 0;
     const int nr = c->nr;
-    if (!(c->list_h > 0.0f) || nr <= 0 || nr > 65535) return 0;
+    if (!(c->list_h > 0.0f) || nr <= 0) return 0;
     const float h = c->list_h;
     const float rad = std::sqrt(WIN2) + 0.5f * std::sqrt(3.0f) * h + 0.01f;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
@@ -1322,7 +1377,8 @@ static int build_lists(lkl_ctx* c) {
         nodes *= n[a];
     }
     if (nodes > LIST_NODES_MAX || (n[1] + NLY - 1) / NLY > 65535 || (n[2] + NLZ - 1) / NLZ > 65535) return 0;
-    const long long entries = fill_lists(c, o, n, h, rad, LIST_PAD, true, &c->d_noff, &c->d_nidx, &c->list_dummy);
+    const long long entries = fill_lists(c, o, n, h, rad, LIST_PAD, true, &c->d_noff, &c->d_nidx, &c->d_nidx32,
+                                         &c->list_dummy);
     if (entries <= 0) return (int)entries;
     for (int a = 0; a < 3; ++a) { c->nn[a] = n[a]; c->no[a] = o[a]; }
     c->list_entries = (size_t)entries;
@@ -1339,7 +1395,8 @@ static int ensure_iface_lists(lkl_ctx* c) {
     free_iface_lists(c);
     const float h = c->list_h;
     const float rad = std::sqrt(15.211f) + 0.5f * std::sqrt(3.0f) * h + 0.01f;
-    const long long entries = fill_lists(c, c->no, c->nn, h, rad, IFACE_PAD, false, &c->d_noff_if, &c->d_nidx_if);
+    const long long entries = fill_lists(c, c->no, c->nn, h, rad, IFACE_PAD, false, &c->d_noff_if, &c->d_nidx_if,
+                                         &c->d_nidx32_if);
     if (entries < 0) return -1;
     if (entries == 0) { set_err("interface lists exceed the entry cap"); return -1; }
     c->iflist_version = c->version;
@@ -1572,6 +1629,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
     if (c->list_entries) {
         C.noff = c->d_noff;
         C.nidx = c->d_nidx;
+        C.nidx32 = c->d_nidx32;
         C.ndum = c->list_dummy;
         C.nnx = c->nn[0];
         C.nny = c->nn[1];
@@ -1611,9 +1669,13 @@ static void ctx_free(lkl_ctx* c) {
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
-                    c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder};
+                    c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder, c->d_nidx32,
+                    c->d_nidx32_if};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    for (hipEvent_t e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1724,9 +1786,33 @@ static size_t tile_lds(size_t nT, bool grouped) { return grouped ? 10 * nT + 16 
 // table and one type byte per sorted atom and the dummy, in whole words.
 static size_t type_lds(const lkl_ctx* c) { return 16 * (size_t)c->ntype + ((size_t)c->nr + 4) / 4 * 4; }
 
+// Device scratch of one launch given by the caller instead of the context's
+// own grown-on-demand buffers: per-pose cost and dispatch order, desolvation
+// minima of the global variant, and the tile accumulators of multi-tile
+// poses (cleared by the launch itself).
+struct LaunchScratch {
+    unsigned* pcost = nullptr;
+    int* porder = nullptr;
+    unsigned* gmin = nullptr;
+    long long* acc = nullptr;
+    unsigned* cnt = nullptr;
+};
+
+// Clears n 64-bit words (a fill kernel: capturable, unlike a memset node).
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+// Desolvation minima of one pose fit in LDS beside the kernel's own arrays.
+static bool des_in_lds(const lkl_ctx* c) {
+    return (size_t)c->nr * sizeof(unsigned) + 1024 <= c->lds_max && !getenv("LKL_FORCE_GLOBAL_MIN");
+}
+
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
-                        double* d_outS, hipStream_t st, bool peratom, const float4* d_lcoords = nullptr) {
+                        double* d_outS, hipStream_t st, bool peratom, const float4* d_lcoords = nullptr,
+                        const LaunchScratch* ext = nullptr) {
     if (N <= 0) return 0;
     if (validate_flags(c, flags)) return -1;
     DevComplex C = dev_view(c);
@@ -1741,14 +1827,13 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     const bool list = C.noff != nullptr;
     if (flags & F_DESOLV) {
         if (!d_outS) { set_err("F_DESOLV requires an outS buffer"); return -1; }
-        const size_t need = (size_t)c->nr * sizeof(unsigned);
         // static LDS of the kernel (reduction slots) needs headroom too
-        if (need + 1024 <= c->lds_max && !getenv("LKL_FORCE_GLOBAL_MIN")) {
+        if (des_in_lds(c)) {
             des = DES_LDS;
-            shm = need;
+            shm = (size_t)c->nr * sizeof(unsigned);
         } else {
             des = DES_GLOBAL;
-            if (dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
+            if (!ext && dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
         }
         // desolvation tiles hold the whole ligand: order the near field only
         // if its arrays fit (8 bytes of alignment after the receptor minima)
@@ -1780,25 +1865,41 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     if (peratom) C.wgrid = nullptr;
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
-    if (nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
+    if (!ext && nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
+    unsigned* gmin = ext ? ext->gmin : c->d_gmin;
+    long long* acc = ext ? ext->acc : c->d_acc;
+    unsigned* cnt = ext ? ext->cnt : c->d_cnt;
     if (fail_at("launch")) { set_err("launch_score stage=launch: injected failure"); return -1; }
+    if (c->timing) (void)hipEventRecord(c->ev[0], st);
+    c->ev_prepass = false;
     // longest-first dispatch when the launch takes a few rounds of the chip's
     // workgroup slots: a short pre-pass estimates each pose's cost and orders
     // each XCD's range (results do not depend on the order)
+    if (ext && nchunks > 1) {  // the caller's accumulators start at zero
+        // one fill over acc (3N words) and cnt, which follows it directly
+        if ((void*)cnt != (void*)(acc + 3 * (size_t)N)) { set_err("scratch: cnt must follow acc"); return -1; }
+        const size_t w = 3 * (size_t)N + ((size_t)N + 1) / 2;
+        hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((w + 255) / 256)), dim3(256), 0, st,
+                           (unsigned long long*)acc, w);
+    }
     if (c->pose_order && list && nchunks == 1 && !peratom && N > POSE_ORDER_MIN && (N + 7) / 8 <= LPT_MAX) {
-        if (dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
-        if (dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
-        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, c->d_pcost);
+        if (!ext && dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
+        if (!ext && dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
+        unsigned* pcost = ext ? ext->pcost : c->d_pcost;
+        int* porder = ext ? ext->porder : c->d_porder;
+        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, pcost);
         int pm = 64;  // threads of the sort: the range rounded up to a power of two, at most 1024
         while (pm < (N + 7) / 8 && pm < 1024) pm <<= 1;
-        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, c->d_pcost, N, c->d_porder);
+        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, pcost, N, porder);
         HIPCK("launch", hipGetLastError());
-        C.porder = c->d_porder;
+        C.porder = porder;
+        c->ev_prepass = true;
     }
+    if (c->timing) (void)hipEventRecord(c->ev[1], st);
     hipLaunchKernelGGL(fn, dim3((unsigned)(N * nchunks)), dim3((unsigned)c->block), shm, st, C,
-                       d_poses, N, d_out, d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_acc,
-                       c->d_cnt, chunk, nchunks);
+                       d_poses, N, d_out, d_outS, gmin, c->d_pa_e, c->d_pa_v, acc, cnt, chunk, nchunks);
     HIPCK("launch", hipGetLastError());
+    if (c->timing) (void)hipEventRecord(c->ev[2], st);
     return 0;
 }
 
@@ -1813,15 +1914,39 @@ static int score_host(lkl_ctx* c, const double* poses, int N, unsigned flags, do
     if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
     if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
     if ((flags & F_DESOLV) && dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
-    HIPCK("upload", hipMemcpyAsync(c->d_poses, poses, sizeof(double) * 7 * (size_t)N,
-                                   hipMemcpyHostToDevice, c->stream));
+    // pinned staging: N x 7 poses in, then N x 2 scores and N desolvation
+    // terms out, in one buffer of N x 7 doubles (7 >= 2 + 1)
+    const size_t stage = (size_t)N * 7;
+    if (stage > c->cap_stage) {
+        if (c->h_stage) (void)hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->cap_stage = 0;
+        HIPCK("alloc", hipHostMalloc((void**)&c->h_stage, sizeof(double) * stage, hipHostMallocDefault));
+        c->cap_stage = stage;
+    }
+    memcpy(c->h_stage, poses, sizeof(double) * 7 * (size_t)N);
+    // a failure after the upload is enqueued waits for the stream first: the
+    // staging buffer is the next call's too
+    const hipError_t up = hipMemcpyAsync(c->d_poses, c->h_stage, sizeof(double) * 7 * (size_t)N,
+                                         hipMemcpyHostToDevice, c->stream);
+    if (up != hipSuccess || fail_at("upload")) (void)hipStreamSynchronize(c->stream);
+    HIPCK("upload", up);
     if (launch_score(c, c->d_poses, N, flags, c->d_out, (flags & F_DESOLV) ? c->d_outS : nullptr,
-                     c->stream, pa_e != nullptr))
+                     c->stream, pa_e != nullptr)) {
+        (void)hipStreamSynchronize(c->stream);
         return -1;
+    }
+    // both copies are waited for before any failure is reported, so no DMA
+    // into the staging buffer outlives the call
+    hipError_t rb = hipMemcpyAsync(c->h_stage, c->d_out, sizeof(double) * 2 * (size_t)N,
+                                   hipMemcpyDeviceToHost, c->stream);
+    if (rb == hipSuccess && (flags & F_DESOLV))
+        rb = hipMemcpyAsync(c->h_stage + 2 * (size_t)N, c->d_outS, sizeof(double) * (size_t)N,
+                            hipMemcpyDeviceToHost, c->stream);
     HIPCK("sync", hipStreamSynchronize(c->stream));
-    HIPCK("readback", hipMemcpy(out, c->d_out, sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToHost));
-    if (flags & F_DESOLV)
-        HIPCK("readback", hipMemcpy(outS, c->d_outS, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost));
+    HIPCK("readback", rb);
+    memcpy(out, c->h_stage, sizeof(double) * 2 * (size_t)N);
+    if (flags & F_DESOLV) memcpy(outS, c->h_stage + 2 * (size_t)N, sizeof(double) * (size_t)N);
     if (pa_e) {
         std::vector<float> e((size_t)N * c->nl), v((size_t)N * c->nl);
         HIPCK("readback", hipMemcpy(e.data(), c->d_pa_e, sizeof(float) * e.size(), hipMemcpyDeviceToHost));
@@ -1896,6 +2021,26 @@ LKL_API int lkl_ctx_score_coords(lkl_ctx* c, const float* coords, int N, unsigne
     HIPCK("readback", hipMemcpy(out, c->d_out, sizeof(double) * 2 * (size_t)N, hipMemcpyDeviceToHost));
     if (flags & F_DESOLV)
         HIPCK("readback", hipMemcpy(outS, c->d_outS, sizeof(double) * (size_t)N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+LKL_API int lkl_ctx_set_timing(lkl_ctx* c, int on) {
+    if (!c) { set_err("lkl_ctx_set_timing: null context"); return -1; }
+    if (on)
+        for (hipEvent_t& e : c->ev)
+            if (!e) HIPCK("alloc", hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    c->timing = on != 0;
+    return 0;
+}
+
+LKL_API int lkl_ctx_kernel_ms(lkl_ctx* c, double* prepass_ms, double* score_ms) {
+    if (!c || !c->timing) { set_err("lkl_ctx_kernel_ms: timing is not enabled on this context"); return -1; }
+    HIPCK("sync", hipEventSynchronize(c->ev[2]));
+    float a = 0.0f, b = 0.0f;
+    HIPCK("sync", hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    HIPCK("sync", hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    if (prepass_ms) *prepass_ms = c->ev_prepass ? a : 0.0;
+    if (score_ms) *score_ms = b;
     return 0;
 }
 
@@ -2069,6 +2214,7 @@ LKL_API int lkl_ctx_interface_async(lkl_ctx* c, const double* d_poses, int N, in
     if (ensure_iface_lists(c) || iface_view(c, &I)) return -1;
     I.noff = c->d_noff_if;
     I.nidx = c->d_nidx_if;
+    I.nidx32 = c->d_nidx32_if;
     const size_t shm = sizeof(unsigned) * (size_t)((c->nr + 31) / 32 + (c->nl + 31) / 32);
     if (shm + 1024 > c->lds_max) { set_err("lkl_ctx_interface_async: interface bitsets exceed LDS"); return -1; }
     hipStream_t st = stream ? (hipStream_t)stream : c->stream;
@@ -2525,9 +2671,19 @@ static std::vector<std::vector<uint64_t>> g_lru;
 static uint64_t bits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 static uint64_t ptr(const void* p) { return (uint64_t)(uintptr_t)p; }
 
+// Least recently used first in g_lru: a hit moves its key to the back, so the
+// complexes a caller alternates between stay resident.
 static lkl_ctx* cache_get(const std::vector<uint64_t>& key, const lkl_complex_desc& d) {
     auto it = g_cache.find(key);
-    if (it != g_cache.end()) return it->second;
+    if (it != g_cache.end()) {
+        auto pos = std::find(g_lru.begin(), g_lru.end(), key);
+        if (pos != g_lru.end() && pos + 1 != g_lru.end()) {
+            std::vector<uint64_t> k = std::move(*pos);
+            g_lru.erase(pos);
+            g_lru.push_back(std::move(k));
+        }
+        return it->second;
+    }
     lkl_ctx* c = ctx_create(&d);
     if (!c) return nullptr;
     if (g_lru.size() >= 8) {  // bound the number of resident complexes

@@ -15,8 +15,17 @@ from ._capi import (F_CLASH, F_DESOLV, F_ELEC, F_FAR, FLAGS_DNA, FLAGS_PYDOCK, F
 FACTOR, EPSILON = 332.0, 4.0
 
 __all__ = ["Context", "Multi", "device_available", "lib", "last_error", "GpuError", "LibraryMissing",
-           "family_flags", "fold_scores", "fold_restraints", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
+           "family_flags", "fold_scores", "tuned_tiles", "fold_restraints", "F_FAR", "F_ELEC", "F_CLASH", "F_DESOLV",
            "FLAGS_DNA", "FLAGS_PYDOCK", "FLAGS_VDW"]
+
+
+def tuned_tiles(key: str):
+    """Stored autotuner choice for a workload key ("c2-dna", "c3-cpydock", "c4",
+    ...) from tiles.json beside this module, or None."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tiles.json")) as f:
+        return json.load(f).get(key)
 
 
 def device_available() -> bool:
@@ -153,6 +162,16 @@ class Context:
                                         _capi.ptr(out, C.c_double), _capi.ptr(s, C.c_double)),
                     "lkl_ctx_score")
         return out[:, 0].copy(), out[:, 1].copy(), s
+
+    def set_timing(self, on):
+        """lkl_ctx_set_timing: HIP events around every launch's kernels."""
+        _capi.check(lib().lkl_ctx_set_timing(self.h, int(bool(on))), "lkl_ctx_set_timing")
+
+    def kernel_ms(self):
+        """(pre-pass ms, scoring-kernel ms) of the last launch (timing on)."""
+        a, b = C.c_double(), C.c_double()
+        _capi.check(lib().lkl_ctx_kernel_ms(self.h, C.byref(a), C.byref(b)), "lkl_ctx_kernel_ms")
+        return a.value, b.value
 
     def set_pose_order(self, on):
         """lkl_ctx_set_pose_order: longest-first dispatch of large launches."""

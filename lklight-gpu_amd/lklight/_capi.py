@@ -113,6 +113,8 @@ def lib():
         "lkl_dataset_poses": (C.c_int, [C.c_uint64, C.c_int, _d]),
         "lkl_ctx_set_pose_order": (C.c_int, [C.c_void_p, C.c_int]),
         "lkl_ctx_pose_order": (C.c_int, [C.c_void_p]),
+        "lkl_ctx_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_ctx_kernel_ms": (C.c_int, [C.c_void_p, _d, _d]),
         "lkl_synth_rsa": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                     C.c_uint64, C.c_int64, _d, C.POINTER(C.c_int64)]),
         "lkl_ctx_build_field": (C.c_int, [C.c_void_p]),

@@ -38,15 +38,29 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB):
+def build_native(out_dir):
+    """Compile the same oracle with -march=native into out_dir (the host it
+    runs on decides the ISA, so this is done on the machine that times it);
+    returns the library path."""
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "liblkl_oracle_native.so")
+    src = [os.path.join(HERE, f) for f in ("lkl_oracle.c", "lkl_oracle_gso.c", "lkl_oracle_sur.c",
+                                           "lkl_oracle_sasa.c")]
+    subprocess.run(["gcc", "-O3", "-march=native", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
+                    "-std=gnu11", "-shared", "-o", out] + src + ["-lm", "-lpthread"], check=True)
+    return out
+
+
+def lib(path=None):
+    """The oracle library (default: oracle/_build/liblkl_oracle.so, -march=x86-64)."""
+    path = path or LIB
+    if path not in _libs:
+        if path == LIB and not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
+        L = C.CDLL(path)
         L.orc_field_dims.restype = C.c_int64
         L.orc_field_dims.argtypes = [_d, C.c_int, _d, C.POINTER(C.c_int64)]
         L.orc_field_fill.restype = None
@@ -116,8 +130,8 @@ def lib():
         L.orc_membrane_intersection.argtypes = [C.POINTER(C.c_ubyte), C.POINTER(C.c_int), C.c_int]
         L.orc_restraint_fold.restype = C.c_double
         L.orc_restraint_fold.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a, t=C.c_double):
@@ -166,7 +180,8 @@ class OField:
 class Oracle:
     """CPU grid path of one complex."""
 
-    def __init__(self, cx):
+    def __init__(self, cx, lib_path=None):
+        self.lib_path = lib_path
         self.rec, self.lig = OMol(cx.rec), OMol(cx.lig)
         self.field = OField(cx.phi, cx.origin, cx.dims) if cx.phi is not None else None
 
@@ -188,7 +203,7 @@ class Oracle:
 
     def batch(self, method, poses, nthreads=1, counts=False):
         """Batched energy_grid -> dict of arrays e, v, s (+ n_near, n_far)."""
-        L = lib()
+        L = lib(self.lib_path)
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
         n = p.shape[0]
         e, v, s = np.zeros(n), np.zeros(n), np.zeros(n)

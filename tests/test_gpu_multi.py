@@ -8,12 +8,10 @@ import pytest
 
 import lklight
 from lklight import synth
+from oracle import lkl_oracle as O
+from scoring_checks import assert_scores, say
 
 pytestmark = pytest.mark.gpu
-
-
-def say(*a):
-    print("[gpu-test]", *a, flush=True)
 
 
 @pytest.fixture(scope="module")
@@ -58,7 +56,7 @@ def test_multi_matches_single(batch):
     m.use_graph(False)
     assert np.array_equal(m.score(allp), ref)
     m.use_graph(True)
-    for block, mult in [(64, 1), (512, 0), (128, 4), (256, 2)]:
+    for block, mult in [(64, 1), (512, 0), (128, 4), (256, 2), (1024, 0), (1024, 1)]:
         say("multi tiles", block, mult)
         m.set_tiles(block, mult)
         assert np.array_equal(m.score(allp), ref), (block, mult)
@@ -72,3 +70,43 @@ def test_multi_rejects_desolvation(batch):
     cxs, ctxs, flags, poses = batch
     with pytest.raises(lklight.GpuError, match="desolvation"):
         lklight.Multi(ctxs[:2], [lklight.F_DESOLV, 0], [4, 4])
+
+
+def test_multi_full_size(gpu):
+    """The 64 complexes at full size (receptors 2k-20k atoms, ligands 500-5k,
+    100-500 glowworms, families mixed; 20,180 poses per step): the batch
+    launch, replayed from its graph, equals the 64 per-complex launches bit
+    for bit at the stored tiles and at 1,024-thread tiles, and the first
+    complex of each family matches the oracle on 8 poses."""
+    say("build the 64 full-size complexes (device-built phi)")
+    cxs = synth.make_config4(64)
+    ctxs, flags, poses = [], [], []
+    for k, cx in enumerate(cxs):
+        fam = cx.meta["family"]
+        ctxs.append(lklight.Context(cx, with_field=False, device_field=(fam != "vdw"), cell_size=4.0))
+        flags.append(lklight.family_flags(fam))
+        poses.append(synth.make_poses(cx.meta["glowworms"], cx.meta["rec_axes"], 500 + k))
+    allp = np.concatenate(poses)
+    assert allp.shape[0] == 20180
+    say("64 sequential launches")
+    ref = np.concatenate([np.stack(ctx.score(p, f)[:2], 1) for ctx, p, f in zip(ctxs, poses, flags)])
+    m = lklight.Multi(ctxs, flags, [p.shape[0] for p in poses])
+    t = lklight.tuned_tiles("c4")
+    for block, mult in [(t["block"], t["chunk_mult"]), (1024, 0), (1024, 1)]:
+        say("batch tiles", block, mult)
+        m.set_tiles(block, mult)
+        for rep in range(2):
+            assert np.array_equal(m.score(allp), ref), (block, mult, rep)
+    m.close()
+    off = np.concatenate([[0], np.cumsum([p.shape[0] for p in poses])])
+    for fam in ("dna", "pydock", "vdw"):
+        k = next(i for i, cx in enumerate(cxs) if cx.meta["family"] == fam)
+        cx = cxs[k]
+        say("oracle:", fam, "complex", k, "(%d x %d atoms)" % (cx.rec.n, cx.lig.n))
+        if fam != "vdw":
+            cx.build_field()  # host copy of the GPU build (bit-equal to the oracle's)
+        got = ref[off[k]:off[k] + 8]
+        assert_scores(fam, {"e": got[:, 0], "v": got[:, 1]}, O.Oracle(cx).batch(fam, poses[k][:8], nthreads=8))
+        cx.phi = None
+    for c in ctxs:
+        c.close()

@@ -15,30 +15,9 @@ import pytest
 import lklight
 from lklight import _capi, synth
 from oracle import lkl_oracle as O
+from scoring_checks import FLOOR, REL, assert_scores, say  # noqa: F401
 
 pytestmark = pytest.mark.gpu
-
-REL = 1e-5
-FLOOR = 1.0
-
-
-def say(*a):
-    print("[gpu-test]", *a, flush=True)
-    sys.stdout.flush()
-
-
-def assert_scores(method, gpu, orc, rel=REL, floor=FLOOR):
-    """gpu/orc: dicts e, v, s -> folded scores + component budgets."""
-    g = lklight.fold_scores(method, gpu["e"], gpu["v"], gpu.get("s"))
-    o = lklight.fold_scores(method, orc["e"], orc["v"], orc.get("s"))
-    tol = rel * np.maximum(np.abs(o), floor)
-    err = np.abs(g - o)
-    say(method, "max rel err %.3e" % float((err / np.maximum(np.abs(o), floor)).max()))
-    assert (err <= tol).all(), (method, err.max(), np.argmax(err / tol))
-    assert (np.abs(gpu["e"] - orc["e"]) * 83.0 <= tol).all()
-    assert (np.abs(gpu["v"] - orc["v"]) <= tol).all()
-    if gpu.get("s") is not None:
-        assert (np.abs(gpu["s"] - orc["s"]) <= tol).all()
 
 
 @pytest.fixture(scope="module")
@@ -116,7 +95,7 @@ def test_cpydock_parity(small):
 
 def test_tile_and_order_invariance(small):
     """Fixed-point per-pose sums: bitwise identical for every cell edge (cubic
-    and narrower along x), block size, ligand tile, neighbour-list spacing (or
+    and narrower along x), block size (64 to 1,024 threads), ligand tile, neighbour-list spacing (or
     the cell-row walk), receptor parameters from the LDS type table or with
     the records, and input atom order."""
     cx, orc, poses = small
@@ -131,7 +110,10 @@ def test_tile_and_order_invariance(small):
     for i, (cell, block, chunk, cell_x, lh) in enumerate(
             [(10.0, 256, 0, 0.0, 1.0), (3.4, 64, 64, 0.0, 0.0), (6.0, 512, 0, 0.0, 0.5),
              (4.0, 128, 256, 0.0, 2.0), (5.0, 64, 128, 0.0, 1.5), (5.0, 256, 0, 1.25, 0.0),
-             (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0)]):
+             (7.5, 128, 64, 2.5, 0.75), (3.4, 512, 0, 0.75, 3.0),
+             # 1,024-thread tiles: the grouped list walk with the LDS type
+             # table and LDS desolvation minima, then the cell-row walk
+             (7.5, 1024, 0, 1.25, 1.0), (5.0, 1024, 256, 0.0, 0.0)]):
         packed = i % 2 == 0
         types = i % 4 < 2
         say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh} packed phi {packed}"
@@ -215,23 +197,44 @@ def test_list_build_counts(gpu):
     ctx.close()
 
 
-def test_large_receptor_row_walk(gpu):
-    """A 70,000-atom receptor exceeds the lists' 16-bit indices: the context
-    falls back to the cell-row walk and still matches the oracle (vdw, no far
-    field, 12 poses)."""
+def test_large_receptor_wide_lists(gpu):
+    """A 70,000-atom receptor: its dummy index (nr) does not fit 16 bits, so
+    the neighbour lists (and the interface pass's short lists) hold 32-bit
+    indices.  The list walk stays in use, matches the oracle for dna (far
+    field built on the device) and vdw on 12 poses, equals the cell-row walk
+    bit for bit, and the interface counts equal the oracle's."""
     say("large receptor: 70,000 atoms")
     ax = synth.scaled_axes(70000, (1.0, 0.95, 0.9))
     rec, _ = synth.make_mol(70000, ax, 123, +10.0)
     lax = synth.scaled_axes(300, (1.0, 1.0, 1.0))
     lig, _ = synth.make_mol(300, lax, 124, -2.0, center=True)
     cx = synth.Complex(rec=rec, lig=lig, meta={"rec_axes": ax})
-    ctx = lklight.Context(cx, with_field=False)
-    assert ctx.list_info()[0] == 0
+    say("large receptor: far field on the device")
+    cx.build_field()
+    ctx = lklight.Context(cx)
+    assert ctx.list_info()[0] > 0
     poses = synth.make_poses(12, ax, 9)
-    got = run(ctx, poses, lklight.FLAGS_VDW)
     orc = O.Oracle(cx)
-    assert_scores("vdw", got, orc.batch("vdw", poses, nthreads=8))
-    assert (got["v"] != 0).sum() >= 6  # most poses touch the receptor
+    got = {}
+    for method in ("dna", "vdw"):
+        say("large receptor:", method)
+        got[method] = run(ctx, poses, lklight.family_flags(method))
+        assert_scores(method, got[method], orc.batch(method, poses, nthreads=8))
+    assert (got["vdw"]["v"] != 0).sum() >= 6  # most poses touch the receptor
+    rng = np.random.default_rng(5)
+    rec_res = [list(range(a, a + 8)) for a in rng.choice(np.arange(0, cx.rec.n - 8, 8), 200, replace=False)]
+    lig_res = [list(range(a, a + 5)) for a in rng.choice(np.arange(0, cx.lig.n - 5, 5), 8, replace=False)]
+    membrane = sorted(rng.choice(cx.rec.n, 300, replace=False).tolist())
+    ctx.set_interface(rec_res, lig_res, membrane)
+    say("large receptor: interface counts")
+    want = _iface_counts_oracle(orc, poses, rec_res, lig_res, membrane)
+    assert np.array_equal(ctx.interface(poses), want)
+    assert want[:, 1].sum() > 0
+    ctx.set_list_spacing(0.0)
+    assert ctx.list_info()[0] == 0
+    say("large receptor: cell-row walk")
+    rows = run(ctx, poses, lklight.FLAGS_DNA)
+    assert np.array_equal(rows["e"], got["dna"]["e"]) and np.array_equal(rows["v"], got["dna"]["v"])
     ctx.close()
 
 
@@ -470,33 +473,66 @@ def test_config1_parity(gpu):
     ctx.close()
 
 
-def test_config2_full_size(gpu):
-    """12,625-atom receptor / 1,500-atom ligand at full size: oracle on a pose
-    subset (phi built on the GPU, bit-equal to the oracle's by
-    test_field_build_bitwise), size-independent properties on the full batch."""
+@pytest.fixture(scope="module")
+def c2(gpu):
     cx = synth.make_config("c2")
     say("c2 field build (GPU)")
     cx.build_field()
+    return cx, O.Oracle(cx)
+
+
+@pytest.mark.parametrize("method", ["dna", "pydock", "vdw"])
+def test_config2_full_size(c2, method):
+    """12,625-atom receptor / 1,500-atom ligand at full size, one 1,200-pose
+    launch per family: the oracle on 30 of its poses (phi built on the GPU,
+    bit-equal to the oracle's by test_field_build_bitwise), size-independent
+    properties on the full batch."""
+    cx, orc = c2
+    flags = lklight.family_flags(method)
     poses = synth.make_poses(1200, cx.meta["rec_axes"], 7)
-    ctx = lklight.Context(cx)
-    say("c2 score 1200 poses")
-    full = run(ctx, poses, lklight.FLAGS_DNA)
-    orc = O.Oracle(cx)
+    ctx = lklight.Context(cx, with_field=method != "vdw")
+    t = lklight.tuned_tiles("c2-" + method)
+    ctx.set_tiles(t["cell"], t["block"], t["chunk"], cell_x=t["cell_x"])
+    say("c2", method, "score 1200 poses")
+    full = run(ctx, poses, flags)
     sub = poses[::40]
     say("c2 oracle subset", sub.shape[0])
-    assert_scores("dna", {"e": full["e"][::40], "v": full["v"][::40]},
-                  orc.batch("dna", sub, nthreads=8))
+    assert_scores(method, {"e": full["e"][::40], "v": full["v"][::40]}, orc.batch(method, sub, nthreads=8))
     # batch split and repeat invariance at full size
-    again = run(ctx, poses[600:], lklight.FLAGS_DNA)
+    again = run(ctx, poses[600:], flags)
     assert np.array_equal(again["e"], full["e"][600:]) and np.array_equal(again["v"], full["v"][600:])
     # the default layouts (neighbour lists, corner-packed phi) against the
     # cell-row walk sampling phi directly: the same bits at full size
-    assert ctx.list_info()[0] > 0 and ctx.phi_packed()
+    assert ctx.list_info()[0] > 0 and (ctx.phi_packed() or method == "vdw")
     ctx.set_list_spacing(0.0)
     ctx.set_phi_packed(False)
-    say("c2 score 1200 poses, cell-row walk and unpacked phi")
-    rows = run(ctx, poses, lklight.FLAGS_DNA)
+    say("c2", method, "score 1200 poses, cell-row walk and unpacked phi")
+    rows = run(ctx, poses, flags)
     assert np.array_equal(rows["e"], full["e"]) and np.array_equal(rows["v"], full["v"])
+    ctx.close()
+
+
+def test_blocking_call_equals_async_and_times_kernels(c2):
+    """The blocking host-buffer call (pinned staging, one synchronisation) and
+    the asynchronous call on device buffers give the same bits, and the
+    context's kernel timing reports both kernels of a longest-first launch."""
+    import torch
+    cx, orc = c2
+    poses = synth.make_poses(1200, cx.meta["rec_axes"], 11)
+    ctx = lklight.Context(cx)
+    ctx.set_timing(True)
+    e, v, _ = ctx.score(poses, lklight.FLAGS_DNA)
+    pre, ker = ctx.kernel_ms()
+    assert pre > 0.0 and ker > 0.0
+    d_p = torch.from_numpy(poses).cuda()
+    d_o = torch.zeros((1200, 2), dtype=torch.float64, device="cuda")
+    ctx.score_async(d_p.data_ptr(), 1200, lklight.FLAGS_DNA, d_o.data_ptr())
+    torch.cuda.synchronize()
+    o = d_o.cpu().numpy()
+    assert np.array_equal(o[:, 0], e) and np.array_equal(o[:, 1], v)
+    # a smaller call after a larger one reuses the staging buffer
+    e2, v2, _ = ctx.score(poses[:17], lklight.FLAGS_DNA)
+    assert np.array_equal(e2, e[:17]) and np.array_equal(v2, v[:17])
     ctx.close()
 
 

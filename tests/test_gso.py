@@ -279,3 +279,104 @@ def test_gso_with_restraints_on_gpu(gpu):
     for g in (gpu_g, cpu_g, dev_g, plain):
         g.close()
     ctx.close()
+
+
+@pytest.fixture(scope="module")
+def c1_dna(gpu):
+    cx = synth.make_config("c1")
+    cx.build_field()
+    pos = synth.make_swarm_positions(2, 20, cx.meta["rec_axes"], 13, lig_radius=cx.meta["lig_axes"][0])
+    orc = O.Oracle(cx)
+
+    def cpu(poses):  # the caller's CPU grid path (dna.rs:1081-1085 re-scores on it)
+        r = orc.batch("dna", poses, nthreads=8)
+        return lklight.fold_scores("dna", r["e"], r["v"])
+    return cx, pos, cpu
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("stage", ["upload", "launch", "sync", "readback"])
+def test_gpu_failure_every_step_uses_the_fallback(c1_dna, stage, monkeypatch):
+    """A GPU context whose every scoring call fails (LKL_FAIL_AT=<stage>, a
+    host-side early return after the stage's own work, no GPU fault) scores
+    every step through the registered CPU fallback: the trajectory equals a
+    fallback-only driver's bit for bit, one fallback call per step that has
+    glowworms to score (a step where none moved makes no call)."""
+    cx, pos, cpu = c1_dna
+    ctx = lklight.Context(cx)
+    steps = 4
+    ref = G.GSO(None, "dna", pos, seed=3, swarm_sizes=[20, 20], fallback=cpu)
+    g = G.GSO(ctx, "dna", pos, seed=3, swarm_sizes=[20, 20], fallback=cpu)
+    monkeypatch.setenv("LKL_FAIL_AT", stage)
+    calls = 0  # steps with glowworms to score (step 0, or some glowworm moved)
+    for s in range(steps):
+        calls += int(s == 0 or g.state()["moved"].any())
+        for d in (g, ref):
+            d.update_luciferin()
+            d.movement_phase()
+        a, b = g.state(), ref.state()
+        for k in ("pose", "scoring", "luciferin", "vision", "neighbors", "moved"):
+            assert np.array_equal(a[k], b[k]), (stage, s, k)
+    monkeypatch.delenv("LKL_FAIL_AT")
+    st = g.stats()
+    assert st["fallback_calls"] == calls >= 3 and st["gpu_launches"] == 0, (st, calls)
+    # the context is healthy again once the injection stops
+    e, v, _ = ctx.score(a["pose"][:4], lklight.FLAGS_DNA)
+    assert np.isfinite(e).all() and np.isfinite(v).all()
+    for d in (g, ref):
+        d.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_failure_mid_run_rescored_on_cpu(c1_dna, monkeypatch):
+    """One failed GPU call in the middle of a run (LKL_FAIL_AT=launch for step
+    3 only): that step's scores are the fallback's (the oracle's values
+    exactly), every other step is GPU-scored, and the trajectory makes the same
+    moves as the run with no failure (poses identical, scores within 1e-5)."""
+    cx, pos, cpu = c1_dna
+    ctx = lklight.Context(cx)
+    g = G.GSO(ctx, "dna", pos, seed=3, swarm_sizes=[20, 20], fallback=cpu)
+    h = G.GSO(ctx, "dna", pos, seed=3, swarm_sizes=[20, 20], fallback=cpu)
+    launches = 0  # GPU-scored steps of g: steps with glowworms to score, but the failed one
+    for s in range(6):
+        launches += int(s != 3 and (s == 0 or g.state()["moved"].any()))
+        if s == 3:
+            monkeypatch.setenv("LKL_FAIL_AT", "launch")
+            before = g.state()
+            need = before["moved"].astype(bool)
+            assert need.any()
+            g.update_luciferin()
+            monkeypatch.delenv("LKL_FAIL_AT")
+            got = g.state()["scoring"]
+            assert np.array_equal(got[need], cpu(before["pose"][need]))
+            g.movement_phase()
+        else:
+            g.update_luciferin()
+            g.movement_phase()
+        h.update_luciferin()
+        h.movement_phase()
+        a, b = g.state(), h.state()
+        assert np.array_equal(a["pose"], b["pose"]) and np.array_equal(a["neighbors"], b["neighbors"]), s
+        err = np.abs(a["scoring"] - b["scoring"]) / np.maximum(np.abs(b["scoring"]), 1.0)
+        assert err.max() <= 1e-5, (s, err.max())
+    assert g.stats()["fallback_calls"] == 1 and g.stats()["gpu_launches"] == launches >= 4, (g.stats(), launches)
+    assert h.stats()["fallback_calls"] == 0
+    for d in (g, h):
+        d.close()
+    ctx.close()
+
+
+@pytest.mark.gpu
+def test_gpu_failure_without_fallback_reports_the_error(c1_dna, monkeypatch):
+    """No fallback registered: the failed GPU call fails the step, and the
+    GPU's message reaches lkl_last_error()."""
+    cx, pos, cpu = c1_dna
+    ctx = lklight.Context(cx)
+    g = G.GSO(ctx, "dna", pos, seed=3, swarm_sizes=[20, 20])
+    monkeypatch.setenv("LKL_FAIL_AT", "readback")
+    with pytest.raises(lklight.GpuError, match="no fallback is registered.*readback"):
+        g.update_luciferin()
+    monkeypatch.delenv("LKL_FAIL_AT")
+    g.close()
+    ctx.close()
