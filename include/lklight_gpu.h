@@ -149,6 +149,22 @@ int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, doub
 int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
                         double* d_out, double* d_outS, void* stream);
 
+/* lkl_ctx_score_async without any allocation (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, results and stream semantics as lkl_ctx_score_async; the
+ * launch's scratch (per-pose dispatch order, tile accumulators, global
+ * desolvation minima) lives in one static 16 MiB device array per GPU instead
+ * of buffers the context grows on demand.  At most about 400,000 poses per
+ * call (fewer with desolvation minima too large for LDS: 4 bytes per receptor
+ * atom per pose); larger calls return -1 with a message.  Every call on a GPU
+ * shares that array, so calls that overlap on one GPU (two streams, two
+ * threads) race: serialise them.  The context's device must be the current
+ * one.  Allocates nothing (the array's address is looked up once per device,
+ * on the first call there). */
+int lkl_ctx_score_async_noalloc(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
+                                double* d_out, double* d_outS, void* stream);
+
 /* Longest-first dispatch (default on): one-tile-per-pose launches of 257 to
  * 16,384 poses over neighbour lists first estimate each pose's near-field
  * cost (list lengths at its atoms' nodes) and start the costly poses first,

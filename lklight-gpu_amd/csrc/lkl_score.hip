@@ -1983,6 +1983,76 @@ LKL_API int lkl_ctx_score_async(lkl_ctx* ctx, const double* d_poses, int N, unsi
     return launch_score(ctx, d_poses, N, flags, d_out, d_outS, st, false);
 }
 
+// ── no-allocation variant of lkl_ctx_score_async ─────────────────────────────
+// For callers that cannot allocate device memory (code under graph capture,
+// serving loops, harnesses that forbid it): the launch's scratch lives in one
+// static device array of SCRATCH_BYTES, shared by every call on a device.
+// Calls that overlap on one device race on it.
+constexpr size_t SCRATCH_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+constexpr int SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_addr[SCRATCH_DEVICES];  // each device's copy, found once
+
+// The address of the current device's copy of `scratch` (NULL and an error
+// message on failure).  Host code never uses the symbol itself as an address.
+static unsigned char* scratch_of(int dev) {
+    if (dev < 0 || dev >= SCRATCH_DEVICES) { set_err("scratch: device %d out of range", dev); return nullptr; }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_err("scratch: hipGetSymbolAddress failed on device %d: %s", dev, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_addr[dev] = (unsigned char*)p;
+    }
+    return g_scratch_addr[dev];
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// Scratch bytes of one no-allocation launch of N poses: cost and order (4 B
+// each), accumulators (24 B) with the counters right after them (4 B, a
+// whole number of 8-byte words), then the global desolvation minima (4 B per
+// receptor atom per pose) when they do not fit in LDS.
+static size_t noalloc_layout(const lkl_ctx* c, int N, unsigned flags, unsigned char* base, LaunchScratch* L) {
+    size_t off = 0;
+    const size_t n = (size_t)N;
+    auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return base ? base + o : nullptr; };
+    L->pcost = (unsigned*)take(4 * n);
+    L->porder = (int*)take(4 * n);
+    unsigned char* a = take(24 * n + 8 * ((n + 1) / 2));
+    L->acc = (long long*)a;
+    L->cnt = a ? (unsigned*)(a + 24 * n) : nullptr;
+    L->gmin = ((flags & F_DESOLV) && !des_in_lds(c)) ? (unsigned*)take(4 * n * (size_t)c->nr) : nullptr;
+    return off;
+}
+
+LKL_API int lkl_ctx_score_async_noalloc(lkl_ctx* ctx, const double* d_poses, int N, unsigned flags,
+                                        double* d_out, double* d_outS, void* stream) {
+    if (!ctx) { set_err("lkl_ctx_score_async_noalloc: null context"); return -1; }
+    if (N < 0) { set_err("N must be >= 0"); return -1; }
+    if (N == 0) return 0;
+    LaunchScratch L;
+    const size_t need = noalloc_layout(ctx, N, flags, nullptr, &L);
+    if (need > SCRATCH_BYTES) {
+        set_err("lkl_ctx_score_async_noalloc: %d poses need %zu bytes of scratch, more than its %zu-byte "
+                "maximum (about %zu poses without global desolvation minima)",
+                N, need, SCRATCH_BYTES, SCRATCH_BYTES / 40);
+        return -1;
+    }
+    int dev = 0;
+    HIPCK("probe", hipGetDevice(&dev));
+    if (dev != ctx->device) { set_err("lkl_ctx_score_async_noalloc: the context lives on device %d, current is %d", ctx->device, dev); return -1; }
+    unsigned char* base = scratch_of(dev);
+    if (!base) return -1;
+    noalloc_layout(ctx, N, flags, base, &L);
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    return launch_score(ctx, d_poses, N, flags, d_out, d_outS, st, false, nullptr, &L);
+}
+
 // The ligand already placed, per pose: coords N x nl x 3 f32 in the ligand's
 // input order.  Each pose gets its own records (sorted order, charge in w) and
 // the identity pose, whose f64 transform returns the f32 inputs exactly, so a

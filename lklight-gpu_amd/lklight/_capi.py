@@ -74,6 +74,8 @@ def lib():
         "lkl_ctx_score": (C.c_int, [C.c_void_p, _d, C.c_int, C.c_uint, _d, _d]),
         "lkl_ctx_score_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint,
                                           C.c_void_p, C.c_void_p, C.c_void_p]),
+        "lkl_ctx_score_async_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_uint,
+                                                  C.c_void_p, C.c_void_p, C.c_void_p]),
         "lkl_ctx_transform": (C.c_int, [C.c_void_p, _d, C.c_int, _d]),
         "lkl_ctx_set_tiles": (C.c_int, [C.c_void_p, C.c_float, C.c_int, C.c_int]),
         "lkl_ctx_autotune": (C.c_int, [C.c_void_p, _d, C.c_int, C.c_uint, _f, _f, _i, _i, _f, _i]),
