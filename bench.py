@@ -403,6 +403,8 @@ def bench_c4(K, W, args, cpu):
                  "gpu_over_cpu": value / (n_cpu / t_cpu), "max_rel_err_vs_cpu": max_rel}
         log("c4: cpu %.0f pose-evals/s, max rel err %.2e" % (cpu_b["value"], max_rel))
     achieved = b_launch / (kern_ms / 1e3) / 1e9 if b_launch else None
+    traffic = measured_traffic({"workload": "c4", "poses_per_launch": P,
+                                "tiles": [tiles["block"], tiles["chunk_mult"]]})
     fams = {str(f): int(n) for f, n in zip(*np.unique([c.meta["family"] for c in cxs], return_counts=True))}
     m.close()
     for c in ctxs:
@@ -417,7 +419,8 @@ def bench_c4(K, W, args, cpu):
         "batch_over_sequential": res["sequential"] / res["batch"],
         "bitwise_equal_to_sequential": same_seq, "host_steps_equal_resident": same_host,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                     "traffic_source": TRAFFIC_FILE if traffic else None,
                      "kernel": "multi_kernel", "kernel_ms": kern_ms, "bytes_per_launch": b_launch},
         "cpu_baseline": cpu_b, "max_rel_err_vs_oracle": max_rel,
     }
