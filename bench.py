@@ -477,7 +477,9 @@ def main():
     ap.add_argument("--sur-glow", type=int, default=50)
     ap.add_argument("--sur-plain-steps", type=int, default=100)
     ap.add_argument("--sur-max-steps", type=int, default=100)
-    ap.add_argument("--sur-grid", type=float, default=0.5)
+    ap.add_argument("--sur-grid", type=float, default=1.0)
+    ap.add_argument("--sur-online", type=int, default=8, help="c5: online Adam steps per screening step")
+    ap.add_argument("--sur-seeds", default="42,7,99", help="c5: start seeds (the first is the headline)")
     ap.add_argument("--sur-trans", type=float, default=2.0)
     ap.add_argument("--sur-rot", type=float, default=0.1)
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -666,12 +668,16 @@ def gso_step_compare(cx, ctx, family, steps=50):
 
 def surrogate_experiment(args):
     """Config-5 comparison on the 1,500-atom / 800-atom complex: train the
-    surrogate on the GPU from exact dna scores, then run plain exact GSO and the
-    surrogate-screened GSO (and the same screen with random picks) from the
-    same start; report rank correlation, top-10 recovery and wall times."""
+    surrogate on the GPU from exact dna scores, then, from the same starts
+    (one set per seed), run plain exact GSO (host-driven and device-resident),
+    the screened GSO with random picks and the surrogate-screened GSO (the
+    network keeps learning from the exact scores of its picks).  Reports rank
+    correlation, top-10 recovery, wall times, the per-candidate costs and the
+    break-even they imply."""
     import numpy as np
     import torch
     import lklight
+    from lklight import gso as G
     from lklight import screen, synth
     from lklight import surrogate as S
 
@@ -682,15 +688,16 @@ def surrogate_experiment(args):
     feats = S.Features(cx, grid=args.sur_grid)
     ns, ng = args.sur_swarms, args.sur_glow
     sizes = [ng] * ns
-    pos = synth.make_swarm_positions(ns, ng, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
+    n = ns * ng
+
+    def starts(seed):
+        return synth.make_swarm_positions(ns, ng, cx.meta["rec_axes"], seed, lig_radius=cx.meta["lig_axes"][0])
     # training poses: candidates along a 100-step plain GSO trajectory from
     # other starts (every k-th step, ~80 %) + random poses in the 15 A shell
     t0 = time.perf_counter()
     n_tr = args.sur_train
-    per_step = ns * ng * 32
-    every = max(1, (100 * per_step) // max(1, (n_tr * 4) // 5))
-    pos_tr = synth.make_swarm_positions(ns, ng, cx.meta["rec_axes"], 4242, lig_radius=cx.meta["lig_axes"][0])
-    traj = screen.collect_poses(ctx, method, pos_tr, 4242, sizes, 100, 32, args.sur_trans, args.sur_rot,
+    every = max(1, (100 * n * 32) // max(1, (n_tr * 4) // 5))
+    traj = screen.collect_poses(ctx, method, starts(4242), 4242, sizes, 100, 32, args.sur_trans, args.sur_rot,
                                 every=every)[:n_tr]
     rnd = synth.make_poses(max(0, n_tr - traj.shape[0]), cx.meta["rec_axes"], 77)
     P = np.concatenate([traj, rnd], 0)
@@ -706,55 +713,121 @@ def surrogate_experiment(args):
     hist = screen.train(net, X[:P.shape[0]], T, epochs=args.sur_epochs, log=log)
     t_train = time.perf_counter() - t0
     del X
-    # held-out: candidates from the benchmark start (different swarms)
+    trained = net.params
     # held-out: candidates at steps 0, 20 and 40 of a plain GSO from the benchmark start
-    hold = screen.collect_poses(ctx, method, pos, 99, sizes, 41, 32, args.sur_trans, args.sur_rot, every=20)
+    hold = screen.collect_poses(ctx, method, starts(42), 99, sizes, 41, 32, args.sur_trans, args.sur_rot, every=20)
     hs = screen.exact_scores(ctx, method, hold)
     hp = screen.predict(net, screen.features_of(feats, hold), hold.shape[0])
     rho = screen.spearman(hp, hs)
     rho_pg = float(np.mean([screen.spearman(hp[i:i + 32], hs[i:i + 32]) for i in range(0, len(hs), 32)]))
     log("held-out spearman %.3f (within-glowworm mean %.3f)" % (rho, rho_pg))
-    # plain exact GSO (the yardstick the screened run is compared with)
-    plain_sc, plain_wall, _ = screen.run_plain(ctx, method, pos, 42, sizes, args.sur_plain_steps)
-    ref10 = screen.top_k(plain_sc)
-    log("plain GSO %d steps: %.3f s, top-10 %s" % (args.sur_plain_steps, plain_wall, np.round(ref10, 2)))
 
-    def screened(picker):
-        sg = screen.ScreenedGSO(ctx, feats, net, method, pos, 42, sizes, trans_radius=args.sur_trans,
-                                rot_sigma=args.sur_rot, picker=picker)
+    # cost of one screening batch (n x 32 candidates) against exact scoring of the same poses
+    nc = n * 32
+    cand = torch.from_numpy(synth.make_poses(nc, cx.meta["rec_axes"], 5)).cuda()
+    Xc = torch.zeros(((nc + 127) // 128 * 128, S.NF), dtype=torch.int16, device="cuda")
+    yc = torch.empty(Xc.shape[0], dtype=torch.float32, device="cuda")
+    d_o = torch.zeros((nc, 2), dtype=torch.float64, device="cuda")
+    cost = {}
+    for rep in range(3):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        reached, steps_done = None, 0
+        feats.compute_device(cand, Xc[:nc], 0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        net.forward_device(Xc, yc)
+        net.sync()
+        t2 = time.perf_counter()
+        ctx.score_async(cand.data_ptr(), nc, lklight.FLAGS_DNA, d_o.data_ptr())
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        cost = {"features_ms": (t1 - t0) * 1e3, "mlp_ms": (t2 - t1) * 1e3, "exact_ms": (t3 - t2) * 1e3}
+    per_cand = (cost["features_ms"] + cost["mlp_ms"]) / nc * 1e6
+    per_exact = cost["exact_ms"] / nc * 1e6
+    log("per candidate: features + MLP %.1f ns, exact score %.1f ns" % (per_cand, per_exact))
+    del Xc, yc, d_o, cand
+
+    def plain_device(pos, seed):
+        g = G.GSO(ctx, method, pos, seed, sizes)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.device_run(args.sur_plain_steps)
+        g.update_luciferin()  # score the last moves
+        wall = time.perf_counter() - t0
+        top = screen.top_k(g.state()["scoring"])
+        g.close()
+        return wall, top
+
+    def screened(pos, seed, picker, ref10, online):
+        net_s = S.MLP(2024)
+        net_s.params = trained
+        sg = screen.ScreenedGSO(ctx, feats, net_s, method, pos, seed, sizes, trans_radius=args.sur_trans,
+                                rot_sigma=args.sur_rot, picker=picker, online_steps=online, target=tmap)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reached, reached_step = None, None
         for k in range(args.sur_max_steps):
             sg.step()
-            steps_done = k + 1
             top = screen.top_k(sg.state()["scoring"])
-            if screen.recovered(top, ref10):
-                reached = time.perf_counter() - t0
-                break
+            if reached is None and screen.recovered(top, ref10):
+                reached, reached_step = time.perf_counter() - t0, k + 1
         wall = time.perf_counter() - t0
-        top = screen.top_k(sg.state()["scoring"])
-        res = {"steps": steps_done, "wall_s": wall, "reached_s": reached, "top10": top.tolist(),
-               "exact_evals": sg.exact_evals, "surrogate_evals": sg.surrogate_evals}
+        res = {"steps": args.sur_max_steps, "wall_s": wall, "ms_per_step": wall / args.sur_max_steps * 1e3,
+               "reached_s": reached, "reached_step": reached_step, "top10": top.tolist(),
+               "exact_evals": sg.exact_evals, "surrogate_evals": sg.surrogate_evals, "online_adam_steps": online,
+               "phase_ms_per_step": {k: v / args.sur_max_steps * 1e3 for k, v in sg.times.items()}}
         sg.close()
+        net_s.close()
         return res
 
-    sur = screened("surrogate")
-    rnd_res = screened("random")
-    log("screened: %s" % {k: v for k, v in sur.items() if k != "top10"})
-    log("random picks: %s" % {k: v for k, v in rnd_res.items() if k != "top10"})
-    # None when the screened run never reached the plain run's top-10 (+-0.5 %)
-    speedup = plain_wall / sur["reached_s"] if sur["reached_s"] else None
-    sur["ms_per_step"] = sur["wall_s"] / sur["steps"] * 1e3
-    rnd_res["ms_per_step"] = rnd_res["wall_s"] / rnd_res["steps"] * 1e3
+    runs = []
+    for seed in [int(v) for v in args.sur_seeds.split(",")]:
+        pos = starts(seed)
+        plain_sc, plain_wall, _ = screen.run_plain(ctx, method, pos, seed, sizes, args.sur_plain_steps)
+        ref10 = screen.top_k(plain_sc)
+        dev_wall, dev_top = plain_device(pos, seed)
+        rnd_res = screened(pos, seed, "random", ref10, 0)
+        sur = screened(pos, seed, "surrogate", ref10, args.sur_online)
+        beats = bool(np.all(np.asarray(sur["top10"]) > np.asarray(rnd_res["top10"])))
+        run = {"seed": seed,
+               "plain": {"steps": args.sur_plain_steps, "wall_s": plain_wall, "top10": ref10.tolist(),
+                         "ms_per_step": plain_wall / args.sur_plain_steps * 1e3},
+               "plain_device": {"wall_s": dev_wall, "top10": dev_top.tolist(),
+                                "ms_per_step": dev_wall / args.sur_plain_steps * 1e3},
+               "random_picks": rnd_res, "screened": sur,
+               "screened_beats_random_at_every_rank": beats,
+               "speedup_vs_plain": plain_wall / sur["reached_s"] if sur["reached_s"] else None,
+               "speedup_vs_plain_device": dev_wall / sur["reached_s"] if sur["reached_s"] else None}
+        if seed == int(args.sur_seeds.split(",")[0]):
+            run["screened_offline_network"] = screened(pos, seed, "surrogate", ref10, 0)
+        log("seed %d: plain %.3f s (device %.3f s) top-10 %s; screened %.3f s reached %s top-10 %s; random top-10 %s; "
+            "beats random at every rank: %s" % (seed, plain_wall, dev_wall, np.round(ref10, 2), sur["wall_s"],
+                                                sur["reached_step"], np.round(sur["top10"], 2),
+                                                np.round(rnd_res["top10"], 2), beats))
+        runs.append(run)
+    head = runs[0]
+    plain_ms = head["plain"]["ms_per_step"]
+    scr_ms = head["screened"]["ms_per_step"]
     out = {
-        "workload": "c5: 1,500-atom receptor x 800-atom ligand, %d swarms x %d glowworms, 32 candidates "
-                    "per glowworm, top 2 exact-scored, grid features h=%.2g A" % (ns, ng, args.sur_grid),
-        "speedup_vs_plain": speedup,
-        "plain": {"steps": args.sur_plain_steps, "wall_s": plain_wall, "top10": ref10.tolist(),
-                  "ms_per_step": plain_wall / args.sur_plain_steps * 1e3},
-        "screened": sur, "random_picks": rnd_res,
+        "workload": "c5: 1,500-atom receptor x 800-atom ligand, %d swarms x %d glowworms, 32 candidates per "
+                    "glowworm, top 2 exact-scored, grid features h=%.2g A, %d online Adam steps per screening "
+                    "step" % (ns, ng, args.sur_grid, args.sur_online),
+        "speedup_vs_plain": head["speedup_vs_plain"],
+        "speedup_vs_plain_device": head["speedup_vs_plain_device"],
+        "seeds": runs,
+        "beats_random_at_every_rank": [r["screened_beats_random_at_every_rank"] for r in runs],
+        "reaches_plain_top10": [r["screened"]["reached_step"] is not None for r in runs],
         "spearman_heldout": rho, "spearman_within_glowworm": rho_pg,
+        "cost_per_batch": dict(cost, candidates=nc),
+        "break_even": {
+            "surrogate_ns_per_candidate": per_cand, "exact_ns_per_pose": per_exact,
+            "candidate_over_exact": per_cand / per_exact,
+            "screened_ms_per_step": scr_ms, "plain_ms_per_step": plain_ms,
+            "plain_device_ms_per_step": head["plain_device"]["ms_per_step"],
+            # a 3x wall-time gain needs the plain run's top-10 within this many screened steps
+            "steps_for_3x_vs_plain": args.sur_plain_steps * plain_ms / (3.0 * scr_ms),
+            "steps_for_3x_vs_plain_device": args.sur_plain_steps * head["plain_device"]["ms_per_step"]
+            / (3.0 * scr_ms)},
         "train": {"poses": int(P.shape[0]), "epochs": args.sur_epochs, "data_s": t_data,
                   "train_s": t_train, "loss": hist},
     }
@@ -762,7 +835,6 @@ def surrogate_experiment(args):
     net.close()
     ctx.close()
     return out
-
 
 
 if __name__ == "__main__":

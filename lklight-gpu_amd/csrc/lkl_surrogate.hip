@@ -482,6 +482,220 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
             }
 }
 
+// ── GEMM, K-contiguous operands: 256 x 128 tiles, split-K ─────────────────────
+// C[M x N] = sum_k A[m * lda + k] * B[n * ldb + k] (both operands K-contiguous:
+// the forward reads the transposed weights, the weight gradients transposed
+// activations).  A workgroup of 4 waves owns a 256 x 128 tile, each wave
+// 128 x 64 = 4 x 2 MFMA tiles of 32 x 32, so a k-step of 16 reads 6 operand
+// fragments from LDS for 8 MFMAs (the 64 x 64 wave tile of gemm_kernel reads
+// 4 for 4, which LDS bandwidth caps near half the MFMA rate).  K advances in
+// steps of 32, double-buffered through registers as in gemm_kernel.
+// Split-K: blockIdx.y selects a chunk of kc columns of K; with more than one
+// chunk each writes its fp32 partial tile to P[z][M][N] and gemm_reduce sums
+// the chunks in ascending z (a fixed order: deterministic) and applies the
+// epilogue.  Rows past M (M a multiple of 128, not of 256) read row M - 1 and
+// are never stored.
+constexpr int BM2 = 256, BN2 = 128, BK2 = 32, LD2 = BK2 + 8;
+constexpr int CH_A2 = BM2 * BK2 / 8 / 256, CH_B2 = BN2 * BK2 / 8 / 256;  // 16-byte chunks per thread
+constexpr int EP_PITCH = 36;  // fp32 pitch of a wave's 32 x 32 epilogue block in LDS
+
+struct Gemm2Args {
+    const bf16* A;
+    const bf16* B;
+    long lda, ldb;
+    int M, N, K, kc;  // kc: K columns per split chunk (a multiple of BK2)
+    float* P;         // split partials [z][M][N], or NULL: epilogue in place
+    float* C32;
+    bf16* C16;
+    long ldc;
+    const float* bias;
+    const bf16* mask;
+};
+
+// Epilogue of 16 consecutive columns n..n+15 of row m (acc values v[16]).
+template <int EPI>
+__device__ __forceinline__ void epi16(const Gemm2Args& g, int m, int n, const float* v) {
+    const long o = (long)m * g.ldc + n;
+    if (EPI == EPI_F32) {
+        float4* d = (float4*)(g.C32 + o);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else if (EPI == EPI_BIAS_RELU) {
+        bf16x8 h[2];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) h[q >> 3][q & 7] = to_bf16(fmaxf(v[q] + g.bias[n + q], 0.0f));
+        *(bf16x8*)(g.C16 + o) = h[0];
+        *(bf16x8*)(g.C16 + o + 8) = h[1];
+    } else {
+        const bf16x8 k0 = *(const bf16x8*)(g.mask + o), k1 = *(const bf16x8*)(g.mask + o + 8);
+        float z[16];
+        bf16x8 h[2];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const float mk = (float)(q < 8 ? k0[q] : k1[q - 8]);
+            z[q] = mk > 0.0f ? v[q] : 0.0f;
+            h[q >> 3][q & 7] = to_bf16(z[q]);
+        }
+        float4* d = (float4*)(g.C32 + o);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d[q] = make_float4(z[4 * q], z[4 * q + 1], z[4 * q + 2], z[4 * q + 3]);
+        *(bf16x8*)(g.C16 + o) = h[0];
+        *(bf16x8*)(g.C16 + o + 8) = h[1];
+    }
+}
+
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm2_kernel(Gemm2Args g) {
+    __shared__ __attribute__((aligned(16))) bf16 As[2][BM2][LD2];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][BN2][LD2];
+    const unsigned ntn = (unsigned)(g.N / BN2), total = ntn * (unsigned)((g.M + BM2 - 1) / BM2);
+    const unsigned L = xcd_tile(blockIdx.x, total);
+    const int m0 = (int)(L / ntn) * BM2, n0 = (int)(L % ntn) * BN2;
+    const int kbeg = (int)blockIdx.y * g.kc;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = (w >> 1) * 128, wn = (w & 1) * 64;
+    f32x16 acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+    // staging: thread t moves 16 bytes of rows t / 4 + 64 i (A: i < 4, B:
+    // i < 2) at column (t % 4) * 8 of each k-step; A rows past M read row M - 1
+    // (named registers: an array here would be kept in scratch memory)
+    static_assert(CH_A2 == 4 && CH_B2 == 2, "four A and two B chunks per thread");
+    const int sr = tid >> 2, sq = (tid & 3) * 8;
+    const bf16* pa0 = g.A + (long)min(m0 + sr, g.M - 1) * g.lda + sq;
+    const bf16* pa1 = g.A + (long)min(m0 + sr + 64, g.M - 1) * g.lda + sq;
+    const bf16* pa2 = g.A + (long)min(m0 + sr + 128, g.M - 1) * g.lda + sq;
+    const bf16* pa3 = g.A + (long)min(m0 + sr + 192, g.M - 1) * g.lda + sq;
+    const bf16* pb0 = g.B + (long)(n0 + sr) * g.ldb + sq;
+    const bf16* pb1 = g.B + (long)(n0 + sr + 64) * g.ldb + sq;
+    uint4 a0, a1, a2, a3, b0, b1;
+#define G2_LOAD(k0)                            \
+    do {                                       \
+        a0 = *(const uint4*)(pa0 + (k0));      \
+        a1 = *(const uint4*)(pa1 + (k0));      \
+        a2 = *(const uint4*)(pa2 + (k0));      \
+        a3 = *(const uint4*)(pa3 + (k0));      \
+        b0 = *(const uint4*)(pb0 + (k0));      \
+        b1 = *(const uint4*)(pb1 + (k0));      \
+    } while (0)
+#define G2_STORE(buf)                          \
+    do {                                       \
+        *(uint4*)&As[buf][sr][sq] = a0;        \
+        *(uint4*)&As[buf][sr + 64][sq] = a1;   \
+        *(uint4*)&As[buf][sr + 128][sq] = a2;  \
+        *(uint4*)&As[buf][sr + 192][sq] = a3;  \
+        *(uint4*)&Bs[buf][sr][sq] = b0;        \
+        *(uint4*)&Bs[buf][sr + 64][sq] = b1;   \
+    } while (0)
+    const int nk = g.kc / BK2;
+    G2_LOAD(kbeg);
+    G2_STORE(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) G2_LOAD(kbeg + (kt + 1) * BK2);
+#pragma unroll
+        for (int kk = 0; kk < BK2; kk += 16) {
+            bf16x8 a[4], b[2];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) a[t] = *(const bf16x8*)&As[cur][wm + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) b[t] = *(const bf16x8*)&Bs[cur][wn + 32 * t + (lane & 31)][kk + 8 * (lane >> 5)];
+#pragma unroll
+            for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+                for (int tj = 0; tj < 2; ++tj)
+                    acc[ti][tj] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ti], b[tj], acc[ti][tj], 0, 0, 0);
+        }
+        if (kt + 1 < nk) G2_STORE(cur ^ 1);  // the other buffer was last read before the previous barrier
+        __syncthreads();
+    }
+#undef G2_LOAD
+#undef G2_STORE
+    // Epilogue through LDS: each wave turns one 32 x 32 block at a time from
+    // the MFMA layout (lane: column lane % 32, rows 8 (r / 4) + 4 (lane / 32) +
+    // r % 4) into rows, so a lane stores 16 consecutive columns of one row.
+    float* eb = reinterpret_cast<float*>(&As[0][0][0]) + w * 32 * EP_PITCH;
+    const int er = lane >> 1, ec = (lane & 1) * 16;
+#pragma unroll
+    for (int ti = 0; ti < 4; ++ti)
+#pragma unroll
+        for (int tj = 0; tj < 2; ++tj) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                eb[(8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)) * EP_PITCH + (lane & 31)] = acc[ti][tj][r];
+            __syncthreads();
+            float v[16];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 t4 = *(const float4*)&eb[er * EP_PITCH + ec + 4 * q];
+                v[4 * q] = t4.x;
+                v[4 * q + 1] = t4.y;
+                v[4 * q + 2] = t4.z;
+                v[4 * q + 3] = t4.w;
+            }
+            __syncthreads();
+            const int m = m0 + wm + 32 * ti + er, n = n0 + wn + 32 * tj + ec;
+            if (m < g.M) {
+                if (g.P) {
+                    float4* d = (float4*)(g.P + ((long)blockIdx.y * g.M + m) * g.N + n);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) d[q] = make_float4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+                } else {
+                    epi16<EPI>(g, m, n, v);
+                }
+            }
+        }
+}
+
+// Sum of the S split partials in ascending z, then the epilogue; one thread
+// per 16 consecutive columns of a row.
+template <int EPI>
+__global__ void __launch_bounds__(256) gemm_reduce(Gemm2Args g, int S) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int per_row = g.N / 16;
+    if (t >= (long)g.M * per_row) return;
+    const int m = (int)(t / per_row), n = (int)(t % per_row) * 16;
+    float v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = 0.0f;
+    for (int z = 0; z < S; ++z) {
+        const float4* p = (const float4*)(g.P + ((long)z * g.M + m) * g.N + n);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 x = p[q];
+            v[4 * q] += x.x;
+            v[4 * q + 1] += x.y;
+            v[4 * q + 2] += x.z;
+            v[4 * q + 3] += x.w;
+        }
+    }
+    epi16<EPI>(g, m, n, v);
+}
+
+// Split-K factor of a K-contiguous GEMM: double it while the launch has fewer
+// than 512 workgroups (two per CU) and each chunk keeps at least 128 columns.
+static int split_of(int M, int N, int K) {
+    const long tiles = (long)((M + BM2 - 1) / BM2) * (N / BN2);
+    int S = 1;
+    while (tiles * S < 512 && K % (2 * S * BK2) == 0 && K / (2 * S) >= 128) S *= 2;
+    return S;
+}
+
+// fp32 workspace (floats) of the split partials of an M x N x K GEMM.
+static size_t split_ws(int M, int N, int K) {
+    const int S = split_of(M, N, K);
+    return S > 1 ? (size_t)S * M * N : 0;
+}
+
+static bool k_contiguous(long sak, long sbk, int M, int N, int K) {
+    return sak == 1 && sbk == 1 && M % 128 == 0 && N % BN2 == 0 && K % BK2 == 0;
+}
+
 // ── small kernels of the MLP ──────────────────────────────────────────────────
 // y[b] = sum_k H[b, k] W4[k] + b4   (512 -> 1 layer)
 __global__ void head_forward(const bf16* __restrict__ H, const bf16* __restrict__ w4, const float* b4,
@@ -845,6 +1059,8 @@ struct lkl_mlp {
     bf16 *h1 = nullptr, *h2 = nullptr, *h3 = nullptr, *dza = nullptr, *dzb = nullptr;  // dz ping-pong
     float *y = nullptr, *dy = nullptr, *dz32 = nullptr, *part = nullptr, *loss = nullptr;
     bf16 *xt = nullptr, *ht = nullptr, *dzt = nullptr;  // transposed operands of the weight-gradient GEMMs
+    float* ws = nullptr;  // split-K partials of the GEMMs
+    size_t ws_cap = 0;    // floats
 };
 
 // Offsets of the transposed W1, W2, W3 in p16t.
@@ -886,12 +1102,33 @@ static int mlp_reserve(lkl_mlp* n, size_t B) {
     return 0;
 }
 
+// Split-K workspace for the GEMMs of a batch of B rows (grown on demand; the
+// old buffer is released only after the device is idle).
+static int mlp_ws(lkl_mlp* n, int B) {
+    const size_t need = std::max(std::max(split_ws(B, D1, NF), split_ws(B, D1, D1)),
+                                 std::max(split_ws(NF, D1, B), split_ws(D1, D1, B)));
+    if (need <= n->ws_cap) return 0;
+    if (n->ws) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(n->ws);
+    }
+    n->ws = nullptr;
+    n->ws_cap = 0;
+    if (hipMalloc((void**)&n->ws, need * 4) != hipSuccess) {
+        n->ws = nullptr;
+        serr("lkl_mlp: split-K workspace allocation failed");
+        return -1;
+    }
+    n->ws_cap = need;
+    return 0;
+}
+
 LKL_API long long lkl_mlp_param_count(void) { return NPARAM; }
 
 LKL_API void lkl_mlp_destroy(lkl_mlp* n) {
     if (!n) return;
     mlp_free_acts(n);
-    void* b[] = {n->p, n->g, n->m, n->v, n->p16, n->p16t, n->loss};
+    void* b[] = {n->p, n->g, n->m, n->v, n->p16, n->p16t, n->loss, n->ws};
     for (void* x : b)
         if (x) (void)hipFree(x);
     if (n->stream) (void)hipStreamDestroy(n->stream);
@@ -967,10 +1204,24 @@ LKL_API int lkl_mlp_get_grads(lkl_mlp* n, float* out) {
     return hipMemcpy(out, n->g, NPARAM * 4, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -1;
 }
 
+// C = op(A) op(B) with an epilogue.  K-contiguous operands take gemm2_kernel
+// (split over K when the tile count is small; ws holds the partials),
+// other strides the general gemm_kernel.
 template <int EPI>
 static void gemm(hipStream_t st, const bf16* A, long sam, long sak, const bf16* B, long sbk, long sbn,
                  int M, int N, int K, float* C32, bf16* C16, long ldc, const float* bias,
-                 const bf16* mask) {
+                 const bf16* mask, float* ws = nullptr) {
+    if (k_contiguous(sak, sbk, M, N, K)) {
+        const int S = split_of(M, N, K);
+        Gemm2Args a = {A, B, sam, sbn, M, N, K, K / S, S > 1 ? ws : nullptr, C32, C16, ldc, bias, mask};
+        const unsigned tiles = (unsigned)(((M + BM2 - 1) / BM2) * (N / BN2));
+        hipLaunchKernelGGL(gemm2_kernel<EPI>, dim3(tiles, (unsigned)S), dim3(256), 0, st, a);
+        if (S > 1) {
+            const long thr = (long)M * (N / 16);
+            hipLaunchKernelGGL(gemm_reduce<EPI>, dim3((unsigned)((thr + 255) / 256)), dim3(256), 0, st, a, S);
+        }
+        return;
+    }
     GemmArgs a = {A, B, sam, sak, sbk, sbn, M, N, K, C32, C16, ldc, bias, mask};
     hipLaunchKernelGGL(gemm_kernel<EPI>, dim3((unsigned)((N / TN) * (M / TM))), dim3(256), 0, st, a);
 }
@@ -982,14 +1233,14 @@ static int check_batch(int B) {
 
 // Forward on device buffers: x (B x 6144 bf16) -> y (B fp32).  B % 128 == 0.
 LKL_API int lkl_mlp_forward(lkl_mlp* n, const void* d_x, int B, float* d_y, void* stream) {
-    if (!n || !d_x || !d_y || check_batch(B) || mlp_reserve(n, B)) return -1;
+    if (!n || !d_x || !d_y || check_batch(B) || mlp_reserve(n, B) || mlp_ws(n, B)) return -1;
     hipStream_t st = stream ? (hipStream_t)stream : n->stream;
     const bf16* X = (const bf16*)d_x;
     const bf16* W = n->p16;
     const bf16* Wt = n->p16t;  // B(k, n) = Wt[n][k]: both operands K-contiguous
-    gemm<EPI_BIAS_RELU>(st, X, NF, 1, Wt + T1, 1, NF, B, D1, NF, nullptr, n->h1, D1, n->p + OB1, nullptr);
-    gemm<EPI_BIAS_RELU>(st, n->h1, D1, 1, Wt + T2, 1, D1, B, D1, D1, nullptr, n->h2, D1, n->p + OB2, nullptr);
-    gemm<EPI_BIAS_RELU>(st, n->h2, D1, 1, Wt + T3, 1, D1, B, D1, D1, nullptr, n->h3, D1, n->p + OB3, nullptr);
+    gemm<EPI_BIAS_RELU>(st, X, NF, 1, Wt + T1, 1, NF, B, D1, NF, nullptr, n->h1, D1, n->p + OB1, nullptr, n->ws);
+    gemm<EPI_BIAS_RELU>(st, n->h1, D1, 1, Wt + T2, 1, D1, B, D1, D1, nullptr, n->h2, D1, n->p + OB2, nullptr, n->ws);
+    gemm<EPI_BIAS_RELU>(st, n->h2, D1, 1, Wt + T3, 1, D1, B, D1, D1, nullptr, n->h3, D1, n->p + OB3, nullptr, n->ws);
     hipLaunchKernelGGL(head_forward, dim3((B + 3) / 4), dim3(256), 0, st, n->h3, W + OW4, n->p + OB4, B, d_y);
     if (hipGetLastError() != hipSuccess) { serr("lkl_mlp_forward: launch failed"); return -1; }
     return 0;
@@ -1005,7 +1256,7 @@ static void colsum(hipStream_t st, const float* A32, const bf16* A16, const floa
 // Loss + gradients (into the object's gradient buffer) on device buffers:
 // x (B x 6144 bf16), t (B fp32 targets).  *loss_out (host) = 0.5 mean (y-t)^2.
 LKL_API int lkl_mlp_grad(lkl_mlp* n, const void* d_x, const float* d_t, int B, float* loss_out) {
-    if (!n || !d_x || !d_t || check_batch(B) || mlp_reserve(n, B)) return -1;
+    if (!n || !d_x || !d_t || check_batch(B) || mlp_reserve(n, B) || mlp_ws(n, B)) return -1;
     hipStream_t st = n->stream;
     if (lkl_mlp_forward(n, d_x, B, n->y, st)) return -1;
     const bf16* X = (const bf16*)d_x;
@@ -1024,20 +1275,20 @@ LKL_API int lkl_mlp_grad(lkl_mlp* n, const void* d_x, const float* d_t, int B, f
         hipLaunchKernelGGL(transpose_kernel, dim3(cols / 64, rows / 64), dim3(256), 0, st, src, rows, cols, dst);
     };
     auto dw = [&](const bf16* Ht, int nin, float* out) {
-        gemm<EPI_F32>(st, Ht, B, 1, n->dzt, 1, B, nin, D1, B, out, nullptr, D1, nullptr, nullptr);
+        gemm<EPI_F32>(st, Ht, B, 1, n->dzt, 1, B, nin, D1, B, out, nullptr, D1, nullptr, nullptr, n->ws);
     };
     // layer 3: dW3 = H2^T dZ3, db3, dZ2 = dZ3 W3^T (H2 > 0)
     tr(n->h2, B, D1, n->ht);
     tr(n->dza, B, D1, n->dzt);
     dw(n->ht, D1, G + OW3);
     colsum(st, n->dz32, nullptr, nullptr, B, D1, n->part, G + OB3);
-    gemm<EPI_MASK>(st, n->dza, D1, 1, W + OW3, 1, D1, B, D1, D1, n->dz32, n->dzb, D1, nullptr, n->h2);
+    gemm<EPI_MASK>(st, n->dza, D1, 1, W + OW3, 1, D1, B, D1, D1, n->dz32, n->dzb, D1, nullptr, n->h2, n->ws);
     // layer 2: dW2 = H1^T dZ2, db2, dZ1 = dZ2 W2^T (H1 > 0)
     tr(n->h1, B, D1, n->ht);
     tr(n->dzb, B, D1, n->dzt);
     dw(n->ht, D1, G + OW2);
     colsum(st, n->dz32, nullptr, nullptr, B, D1, n->part, G + OB2);
-    gemm<EPI_MASK>(st, n->dzb, D1, 1, W + OW2, 1, D1, B, D1, D1, n->dz32, n->dza, D1, nullptr, n->h1);
+    gemm<EPI_MASK>(st, n->dzb, D1, 1, W + OW2, 1, D1, B, D1, D1, n->dz32, n->dza, D1, nullptr, n->h1, n->ws);
     // layer 1: dW1 = X^T dZ1, db1
     tr(X, B, NF, n->xt);
     tr(n->dza, B, D1, n->dzt);
@@ -1106,7 +1357,10 @@ LKL_API int lkl_mlp_grad_host(lkl_mlp* n, const unsigned short* x, const float* 
 LKL_API int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, long sak,
                                const unsigned short* B, long b_len, long sbk, long sbn, int M, int N,
                                int K, float* C) {
-    if (M % TM || N % TN || K % TK || M <= 0 || N <= 0 || K <= 0) { serr("lkl_gemm_bf16_host: shape"); return -1; }
+    if (M % TM || N % TN || (K % TK && !k_contiguous(sak, sbk, M, N, K)) || M <= 0 || N <= 0 || K <= 0) {
+        serr("lkl_gemm_bf16_host: shape");
+        return -1;
+    }
     // one unit stride per operand, the other a multiple of 8 (16-byte vector
     // loads), and every element the kernel reads inside the given lengths
     auto strides_ok = [](long s_row, long s_k) {
@@ -1119,18 +1373,23 @@ LKL_API int lkl_gemm_bf16_host(const unsigned short* A, long a_len, long sam, lo
         return -1;
     }
     void *dA = nullptr, *dB = nullptr;
-    float* dC = nullptr;
+    float *dC = nullptr, *dW = nullptr;
     int rc = 0;
-    if (hipMalloc(&dA, a_len * 2) || hipMalloc(&dB, b_len * 2) || hipMalloc((void**)&dC, (size_t)M * N * 4)) rc = -1;
+    const size_t wsn = k_contiguous(sak, sbk, M, N, K) ? split_ws(M, N, K) : 0;
+    if (hipMalloc(&dA, a_len * 2) || hipMalloc(&dB, b_len * 2) || hipMalloc((void**)&dC, (size_t)M * N * 4) ||
+        (wsn && hipMalloc((void**)&dW, wsn * 4)))
+        rc = -1;
     if (!rc && (hipMemcpy(dA, A, a_len * 2, hipMemcpyHostToDevice) || hipMemcpy(dB, B, b_len * 2, hipMemcpyHostToDevice))) rc = -1;
     if (!rc) {
-        gemm<EPI_F32>(nullptr, (const bf16*)dA, sam, sak, (const bf16*)dB, sbk, sbn, M, N, K, dC, nullptr, N, nullptr, nullptr);
+        gemm<EPI_F32>(nullptr, (const bf16*)dA, sam, sak, (const bf16*)dB, sbk, sbn, M, N, K, dC, nullptr, N, nullptr,
+                      nullptr, dW);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = -1;
     }
     if (!rc && hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost)) rc = -1;
     if (dA) (void)hipFree(dA);
     if (dB) (void)hipFree(dB);
     if (dC) (void)hipFree(dC);
+    if (dW) (void)hipFree(dW);
     return rc;
 }
 

@@ -10,7 +10,14 @@ One step, for every glowworm of every swarm:
      predicted score (lkl_screen_select);
   4. the picks are exact-scored by the scoring kernel; a glowworm takes its best
      pick (accept="best") or takes it only when it beats its current exact
-     score (accept="improve").
+     score (accept="improve");
+  5. with online_steps > 0 the network keeps learning from those exact scores:
+     the picks' features (already computed) and targets go into a replay
+     buffer on the device and online_steps Adam steps of 4,096 rows drawn from
+     it follow every screening step.  The offline-trained network ranks the
+     candidates of poor poses well but not those near a swarm's optima, which
+     the swarm reaches only during the run; the replay buffer holds exactly
+     those.
 The picker "random" replaces step 3's ranking by random predictions (the same
 procedure without the network), for comparison.
 
@@ -123,7 +130,8 @@ class ScreenedGSO:
 
     def __init__(self, ctx, feats, net, method, positions, seed, swarm_sizes=None, n_cand=32,
                  n_exact=2, trans_radius=1.0, rot_sigma=0.05, picker="surrogate", cand_seed=None,
-                 accept="improve"):
+                 accept="improve", online_steps=0, target=None, online_lr=2e-4, replay=65536,
+                 online_batch=4096):
         self.gso = GSO(ctx, method, positions, seed, swarm_sizes)
         self.ctx, self.feats, self.net, self.method = ctx, feats, net, method
         self.n, self.C, self.k = self.gso.n, int(n_cand), int(n_exact)
@@ -148,13 +156,44 @@ class ScreenedGSO:
         self.flags = family_flags(method)
         self.exact_evals = 0
         self.surrogate_evals = 0
+        self.online_steps = int(online_steps) if picker == "surrogate" else 0
+        if self.online_steps:
+            if target is None:
+                raise ValueError("online fine-tuning needs the training target map")
+            self.target, self.lr, self.batch = target, float(online_lr), int(online_batch)
+            self.Xr = torch.zeros((int(replay), NF), dtype=torch.int16, device=dev)
+            self.Tr = torch.zeros(int(replay), dtype=torch.float32, device=dev)
+            self.fill = self.head = 0
+            self.gen = torch.Generator().manual_seed(self.seed)
+        self.times = {"screen": 0.0, "exact": 0.0, "online": 0.0, "host": 0.0}
+
+    def _learn(self, scores):
+        """Add the step's exact-scored picks to the replay buffer and take
+        online_steps Adam steps on batches drawn from it."""
+        m = self.n * self.k
+        rows = (torch.arange(self.n, device="cuda", dtype=torch.int64)[:, None] * self.C
+                + self.d_pick.view(self.n, self.k).long()).reshape(-1)
+        at = (torch.arange(m, device="cuda") + self.head) % self.Xr.shape[0]
+        self.Xr[at] = self.X.index_select(0, rows)
+        self.Tr[at] = torch.from_numpy(self.target(scores.reshape(-1))).cuda()
+        self.head = (self.head + m) % self.Xr.shape[0]
+        self.fill = min(self.Xr.shape[0], self.fill + m)
+        for _ in range(self.online_steps):
+            bi = torch.randint(0, self.fill, (self.batch,), generator=self.gen).cuda()
+            xb, tb = self.Xr.index_select(0, bi), self.Tr.index_select(0, bi)
+            torch.cuda.synchronize()
+            self.net.grad_device(xb, tb, want_loss=False)
+            self.net.adam(self.lr)
+            self.net.sync()  # xb / tb are torch allocations: keep them until the kernels are done
 
     def step(self):
         L = _capi.lib()
         st = self.stream.cuda_stream
+        t0 = time.perf_counter()
         self.gso.update_luciferin()
         base = self.gso.plan()
         cur = self.gso.state()["scoring"]
+        t1 = time.perf_counter()
         with torch.cuda.stream(self.stream):
             self.d_base.copy_(torch.from_numpy(base), non_blocking=False)
             seed = (self.seed * 1000003 + self.step_no) & 0xFFFFFFFFFFFFFFFF
@@ -170,13 +209,19 @@ class ScreenedGSO:
             _capi.check(L.lkl_screen_select(self.y.data_ptr(), self.d_cand.data_ptr(), self.n, self.C, self.k,
                                             self.d_pick.data_ptr(), self.d_sel.data_ptr(), st),
                         "lkl_screen_select")
+            self.stream.synchronize()
+            t2 = time.perf_counter()
             self.ctx.score_async(self.d_sel.data_ptr(), self.n * self.k, self.flags, self.d_out.data_ptr(),
                                  None, st)
             sel = self.d_sel.cpu().numpy().reshape(self.n, self.k, 7)
             out = self.d_out.cpu().numpy()
         self.stream.synchronize()
+        t3 = time.perf_counter()
         self.exact_evals += self.n * self.k
         sc = fold_scores(self.method, out[:, 0], out[:, 1]).reshape(self.n, self.k)
+        if self.online_steps:
+            self._learn(sc)
+        t4 = time.perf_counter()
         best = np.argmax(sc, axis=1)
         bsc = sc[np.arange(self.n), best]
         bpose = sel[np.arange(self.n), best]
@@ -186,6 +231,11 @@ class ScreenedGSO:
             accept = np.ones(self.n, np.uint8)
         self.gso.commit(bpose, bsc, accept)
         self.step_no += 1
+        t5 = time.perf_counter()
+        self.times["host"] += (t1 - t0) + (t5 - t4)
+        self.times["screen"] += t2 - t1
+        self.times["exact"] += t3 - t2
+        self.times["online"] += t4 - t3
         return accept.mean()
 
     def state(self):

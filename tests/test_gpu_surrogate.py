@@ -42,6 +42,27 @@ def test_gemm_bf16_layouts(gpu, layout):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape", [(384, 256, 1024), (384, 256, 1056), (1024, 512, 6144)])
+def test_gemm_split_k(gpu, shape):
+    """K-contiguous operands take the 256 x 128-tile kernel: split over K into
+    8, 2 and 1 chunks for these shapes (partials summed in a fixed order), a
+    last row tile half past M when M = 384.  Same f64 comparison and bound as
+    above."""
+    rng = np.random.default_rng(11)
+    Mm, N, K = shape
+    A = rng.normal(size=(Mm, K))
+    B = rng.normal(size=(K, N))
+    Ab, Bb = _bits(A), _bits(B)
+    Av, Bv = M.bf16_to_f64(Ab), M.bf16_to_f64(Bb)
+    C = S.gemm_bf16(Ab, K, 1, np.ascontiguousarray(Bb.T), 1, K, Mm, N, K)
+    ref = Av @ Bv
+    bound = 2.0 ** -22 * K * (np.abs(Av) @ np.abs(Bv))
+    assert (np.abs(C - ref) <= bound).all(), np.abs(C - ref).max()
+    # deterministic: the same bits on a second call
+    assert np.array_equal(C, S.gemm_bf16(Ab, K, 1, np.ascontiguousarray(Bb.T), 1, K, Mm, N, K))
+
+
+@pytest.mark.gpu
 def test_gemm_rejects_bad_shapes(gpu):
     a = np.zeros(128 * 64, np.uint16)
     with pytest.raises(Exception):
