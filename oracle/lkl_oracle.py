@@ -379,3 +379,39 @@ def feat_grid_counts(rec_xyz, rec_group, lig_xyz, lig_group, poses, h, nthreads=
                                lx.shape[0], _p(P), P.shape[0], float(h), _p(out, C.c_uint32),
                                int(nthreads))
     return out
+
+
+# ── screen candidates (own definition; no reference counterpart) ─────────────
+# A restatement of lkl_screen_candidates: candidate i = g * C + c of glowworm
+# g; c = 0 copies the base pose, c > 0 adds trans_radius * U(-1, 1) to each
+# translation component and rot_sigma * U(-1, 1) to each quaternion component,
+# then renormalises the quaternion.  U = (splitmix64 output >> 11) * 2^-53 with
+# key = mix(seed ^ mix(8 i + d)) for draw d of candidate i, mix being
+# splitmix64's output function applied to z + 0x9E3779B97F4A7C15 (so mix(0) is
+# the first output of splitmix64 seeded with 0, 0xE220A8397B1DCDAF).
+
+def splitmix64_mix(z):
+    """numpy uint64 array -> splitmix64 output of state z (wrapping)."""
+    z = np.asarray(z, np.uint64)
+    with np.errstate(over="ignore"):
+        z = z + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def screen_candidates(base, C, trans_radius, rot_sigma, seed):
+    base = np.asarray(base, np.float64).reshape(-1, 7)
+    n = base.shape[0]
+    i = np.arange(n * C, dtype=np.uint64)
+    out = np.repeat(base, C, axis=0).copy()
+    with np.errstate(over="ignore"):
+        u = np.stack([(splitmix64_mix(np.uint64(seed) ^ splitmix64_mix(i * np.uint64(8) + np.uint64(d)))
+                       >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) for d in range(7)], 1)
+    q = out[:, :4] + rot_sigma * (2.0 * u[:, :4] - 1.0)
+    nrm = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
+    t = out[:, 4:] + trans_radius * (2.0 * u[:, 4:] - 1.0)
+    pert = (i % np.uint64(C)) != 0
+    out[pert, :4] = q[pert] / nrm[pert, None]
+    out[pert, 4:] = t[pert]
+    return out

@@ -13,30 +13,6 @@ from lklight.gso import GSO
 from oracle import lkl_oracle as O
 
 
-def _mix64(z):
-    z = (z + np.uint64(0x9E3779B97F4A7C15))
-    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-    return z ^ (z >> np.uint64(31))
-
-
-def _candidates_host(base, C, tr, rs, seed):
-    """Restatement of lkl_screen_candidates (include/lklight_gpu.h)."""
-    n = base.shape[0]
-    i = np.arange(n * C, dtype=np.uint64)
-    out = np.repeat(base, C, axis=0).copy()
-    with np.errstate(over="ignore"):
-        u = np.stack([(_mix64(np.uint64(seed) ^ _mix64(i * np.uint64(8) + np.uint64(d))) >> np.uint64(11))
-                      .astype(np.float64) * (1.0 / 9007199254740992.0) for d in range(7)], 1)
-    q = out[:, :4] + rs * (2.0 * u[:, :4] - 1.0)
-    nrm = np.sqrt(q[:, 0] * q[:, 0] + q[:, 1] * q[:, 1] + q[:, 2] * q[:, 2] + q[:, 3] * q[:, 3])
-    t = out[:, 4:] + tr * (2.0 * u[:, 4:] - 1.0)
-    pert = (i % np.uint64(C)) != 0
-    out[pert, :4] = q[pert] / nrm[pert, None]
-    out[pert, 4:] = t[pert]
-    return out
-
-
 @pytest.mark.gpu
 def test_candidates_and_select_match_host(gpu):
     rng = np.random.default_rng(3)
@@ -48,7 +24,7 @@ def test_candidates_and_select_match_host(gpu):
     _capi.check(L.lkl_screen_candidates(d_base.data_ptr(), n, C, 2.0, 0.1, 12345, d_cand.data_ptr(), None), "cand")
     torch.cuda.synchronize()
     got = d_cand.cpu().numpy()
-    ref = _candidates_host(base, C, 2.0, 0.1, 12345)
+    ref = O.screen_candidates(base, C, 2.0, 0.1, 12345)
     assert np.abs(got - ref).max() <= 4e-16 * max(1.0, np.abs(ref).max())
     assert np.array_equal(got[::C], base)
     # selection: top-k per glowworm, ties to the lower index
@@ -78,8 +54,9 @@ def trained():
     sc = np.concatenate([screen.exact_scores(ctx, "dna", P[i:i + 65536]) for i in range(0, len(P), 65536)])
     X = screen.features_of(feats, P)
     net = S.MLP(7)
-    hist = screen.train(net, X[:len(P)], torch.from_numpy(screen.Target(sc)(sc)).cuda(), epochs=6, batch=2048)
-    yield cx, ctx, feats, net, hist
+    tmap = screen.Target(sc)
+    hist = screen.train(net, X[:len(P)], torch.from_numpy(tmap(sc)).cuda(), epochs=6, batch=2048)
+    yield cx, ctx, feats, net, hist, tmap
     net.close()
     feats.close()
     ctx.close()
@@ -90,7 +67,7 @@ def test_trained_surrogate_ranks_candidates(gpu, trained):
     """Held-out candidates (other starts): overall and within-glowworm rank
     correlation, and the top pick of each glowworm's 32 beats the average
     candidate in exact score (the network's gain over picking at random)."""
-    cx, ctx, feats, net, hist = trained
+    cx, ctx, feats, net, hist, _ = trained
     assert hist[-1] < 0.5 * hist[0]
     pos = synth.make_swarm_positions(10, 20, cx.meta["rec_axes"], 42, lig_radius=cx.meta["lig_axes"][0])
     hold = screen.collect_poses(ctx, "dna", pos, 99, [20] * 10, 21, 32, 2.0, 0.1, every=10)
@@ -108,7 +85,7 @@ def test_screened_gso_invariants(gpu, trained):
     """Committed scores are the exact scores of the committed poses; with
     accept='improve' no glowworm's score decreases; the plan consumed the same
     draws as a plain movement phase (same vision ranges)."""
-    cx, ctx, feats, net, _ = trained
+    cx, ctx, feats, net, _, _ = trained
     pos = synth.make_swarm_positions(4, 16, cx.meta["rec_axes"], 5, lig_radius=cx.meta["lig_axes"][0])
     sg = screen.ScreenedGSO(ctx, feats, net, "dna", pos, 5, [16] * 4, trans_radius=2.0, rot_sigma=0.1)
     plain = GSO(ctx, "dna", pos, 5, [16] * 4)
@@ -131,3 +108,36 @@ def test_screened_gso_invariants(gpu, trained):
     sg.close()
     sg2.close()
     plain.close()
+
+
+@pytest.mark.gpu
+def test_online_learning_fits_the_screens_exact_scores(gpu, trained):
+    """The screen's online Adam steps train on the exact scores of its own
+    picks (a replay buffer on the device): after 20 screened steps the
+    updated network predicts the buffered picks' targets with a smaller mean
+    squared error than the offline network does, and the run's committed
+    scores are still exact."""
+    cx, ctx, feats, net, _, tmap = trained
+    pos = synth.make_swarm_positions(10, 20, cx.meta["rec_axes"], 11, lig_radius=cx.meta["lig_axes"][0])
+    net_s = S.MLP(7)
+    net_s.params = net.params
+    sg = screen.ScreenedGSO(ctx, feats, net_s, "dna", pos, 11, [20] * 10, trans_radius=2.0, rot_sigma=0.1,
+                            online_steps=4, target=tmap)
+    for _ in range(20):
+        sg.step()
+    st = sg.state()
+    assert np.array_equal(screen.exact_scores(ctx, "dna", st["pose"]), st["scoring"])
+    n = sg.fill
+    assert n == 20 * 200 * 2
+    npad = (n + 127) // 128 * 128
+    X = torch.zeros((npad, S.NF), dtype=torch.int16, device="cuda")
+    X[:n] = sg.Xr[:n]
+    t = sg.Tr[:n].cpu().numpy()
+    mse = {}
+    for name, m in (("offline", net), ("online", net_s)):
+        mse[name] = float(np.mean((screen.predict(m, X, n) - t) ** 2))
+    print("[gpu-test] mse on the screen's exact-scored picks: offline %.4f, online %.4f"
+          % (mse["offline"], mse["online"]))
+    assert mse["online"] < 0.9 * mse["offline"], mse
+    sg.close()
+    net_s.close()

@@ -380,3 +380,57 @@ def test_gpu_failure_without_fallback_reports_the_error(c1_dna, monkeypatch):
     monkeypatch.delenv("LKL_FAIL_AT")
     g.close()
     ctx.close()
+
+
+def test_gso_rules_closed_form():
+    """One movement phase on four glowworms whose outcome follows from
+    glowworm.rs / swarm.rs by hand (scores from a fixed function of the pose
+    through the fallback scorer; initial luciferin 5, vision 0.2, rho 0.5,
+    gamma 0.4, beta 0.08, 5 neighbours, steps 0.5):
+      A (0, 0, 0), B (0.1, 0, 0), C (0, 0.15, 0), D (-0.2, 0, 0) with scores
+      0, 1, 0.9, 28 -> luciferin 2.5, 2.9, 2.86, 13.7;
+      A's neighbours are B and C (brighter, closer than 0.2); D is brighter
+      but exactly 0.2 away, outside the strict `<` of is_neighbor;
+      A picks B when its draw is below (2.9 - 2.5) / (0.4 + 0.36), else C;
+      C's only neighbour is B (0.18 away); B and D have none;
+      a move is 0.5 A towards the partner's pre-move position and slerp(0.5)
+      of the rotations, which for unit quaternions at angle w is the rotation
+      at angle w / 2; vision = 0.2 + 0.08 (5 - |neighbours|).
+    The same test pins the oracle's GSO, which equals the product driver bit
+    for bit (test_gso_logic_matches_oracle_on_cpu)."""
+    half = lambda a: [math.cos(a), math.sin(a), 0.0, 0.0]  # rotation about x by 2a
+    qb, qc = half(0.3), [math.cos(0.2), 0.0, math.sin(0.2), 0.0]
+    pos = np.array([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0],
+                    [0.1, 0.0, 0.0] + qb,
+                    [0.0, 0.15, 0.0] + qc,
+                    [-0.2, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]])
+
+    def scores(poses):  # rows (w, x, y, z, tx, ty, tz)
+        tx, ty = poses[:, 4], poses[:, 5]
+        return 10.0 * tx + 6.0 * ty + 30.0 * (tx < -0.1)
+
+    for seed in (1, 5, 9, 13):
+        g = G.GSO(None, "vdw", pos, seed=seed, fallback=scores)
+        g.update_luciferin()
+        st0 = g.state()
+        assert np.allclose(st0["luciferin"], [2.5, 2.9, 2.86, 13.7], rtol=0, atol=1e-12)
+        g.movement_phase()
+        st = g.state()
+        r = G.rng_f64s(seed, 4)[0]  # the step's first draw is glowworm A's
+        p_b = (2.9 - 2.5) / ((2.9 - 2.5) + (2.86 - 2.5))
+        assert list(st["neighbors"]) == [2, 0, 1, 0]
+        assert np.allclose(st["vision"], [0.44, 0.6, 0.52, 0.6], rtol=0, atol=1e-15)
+        assert list(st["moved"]) == [1, 0, 1, 0]
+        # A: 0.5 A along +x towards B, rotation half-way (angle 0.15), or towards C
+        if r < p_b:
+            assert np.allclose(st["pose"][0], half(0.15) + [0.5, 0.0, 0.0], rtol=0, atol=1e-15)
+        else:
+            assert np.allclose(st["pose"][0], [math.cos(0.1), 0.0, math.sin(0.1), 0.0, 0.0, 0.5, 0.0],
+                               rtol=0, atol=1e-15)
+        # C: towards B's pre-move position; B and D stay
+        d = np.array([0.1, -0.15, 0.0])
+        tc = np.array([0.0, 0.15, 0.0]) + 0.5 * d / np.linalg.norm(d)
+        assert np.allclose(st["pose"][2, 4:], tc, rtol=0, atol=1e-15)
+        assert np.allclose(st["pose"][1], qb + [0.1, 0.0, 0.0], rtol=0, atol=1e-15)
+        assert np.allclose(st["pose"][3], [1.0, 0.0, 0.0, 0.0, -0.2, 0.0, 0.0], rtol=0, atol=1e-15)
+        g.close()

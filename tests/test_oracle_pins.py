@@ -230,3 +230,34 @@ def test_fold_scores():
     assert O.fold("cpydock", 2.0, 3.0, 5.0) == -(2.0 * 83.0 + 0.3 - 5.0)
     assert fold_scores("vdw", 0.0, 3.0) == -3.0
     assert fold_scores("cpydock", 2.0, 3.0, 5.0) == -(166.0 + 0.30000000000000004) + 5.0
+
+
+def test_screen_candidates_oracle_pins():
+    """The oracle's restatement of lkl_screen_candidates: its mix is
+    splitmix64's output function (seed-0 sequence of the published generator:
+    e220a8397b1dcdaf, 6e789e6aa1b965f4, 06c45d188009454f); candidate 0 of each
+    glowworm is its base pose; perturbed quaternions are unit and lie within
+    rot_sigma of the base per component before normalisation (so within a
+    bounded angle of it); translations lie
+    within trans_radius; the draws are uniform (mean 0, variance 1/3 of the
+    radius squared over 20,000 candidates)."""
+    golden = 0x9E3779B97F4A7C15
+    seq = [int(O.splitmix64_mix(np.uint64((k * golden) % 2 ** 64))) for k in range(3)]
+    assert seq == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    rng = np.random.default_rng(2)
+    n, C, tr, rs = 625, 32, 2.0, 0.1
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    base = np.concatenate([q, rng.uniform(-20, 20, (n, 3))], 1)
+    c = O.screen_candidates(base, C, tr, rs, 77).reshape(n, C, 7)
+    assert np.array_equal(c[:, 0], base)
+    p = c[:, 1:]
+    assert np.allclose(np.linalg.norm(p[..., :4], axis=-1), 1.0, atol=1e-15)
+    d = p[..., 4:] - base[:, None, 4:]
+    assert np.abs(d).max() <= tr and abs(d.mean()) < 0.02 and abs(d.var() - tr * tr / 3) < 0.02
+    # a perturbation of at most rs per component of a unit quaternion keeps
+    # the normalised candidate within cos > (1 - 2 rs) / (1 + 2 rs) of it
+    cosang = np.einsum("ijk,ijk->ij", p[..., :4], np.broadcast_to(base[:, None, :4], p[..., :4].shape))
+    assert (cosang > (1 - 2 * rs) / (1 + 2 * rs)).all()
+    assert np.array_equal(O.screen_candidates(base, C, tr, rs, 77), c.reshape(-1, 7))
+    assert not np.array_equal(O.screen_candidates(base, C, tr, rs, 78), c.reshape(-1, 7))
