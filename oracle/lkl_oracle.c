@@ -169,12 +169,14 @@ static int field_in_box(const orc_field* f, double x, double y, double z) {
 
 /* grid_dna.rs:201-209 */
 static double far_field_energy(const orc_field* f, const double* lig_c, const double* q,
-                               int nl, int64_t* n_far) {
+                               int nl, int64_t* n_far, double* f_abs) {
     double e = 0.0;
     for (int j = 0; j < nl; ++j) {
         if (q[j] != 0.0) {
             const double* c = lig_c + 3 * j;
-            e += q[j] * orc_field_sample(f, c[0], c[1], c[2]);
+            const double t = q[j] * orc_field_sample(f, c[0], c[1], c[2]);
+            e += t;
+            if (f_abs) *f_abs += fabs(t);
             if (n_far && field_in_box(f, c[0], c[1], c[2])) ++*n_far;
         }
     }
@@ -364,49 +366,70 @@ void orc_transform(const double* base, int nl, const double* pose7, double* out)
 
 /* dna.rs:951-969: clamped electrostatics + capped LJ (sqrt-product epsilon) +
  * heavy-atom clash. */
+/* Rounding magnitudes of one pair (m[0] += |elec|, m[1] += LJ magnitude,
+ * m[2] += clash weight * distance): not part of the reference's sums, they
+ * size the FP32 error bound of a device evaluation (orc_parts). */
+static void mag_elec(double* m, double e) { if (m) m[0] += fabs(e); }
+static void mag_lj(double* m, double eps, double p6, double k) {
+    if (m && k <= 2.0 * VDW_CUTOFF) m[1] += fabs(eps) * (p6 * p6 + 2.0 * p6);
+}
+
 static void dna_pair(double d2, double qj, double qi, double svj, double svi, double rj,
-                     double ri, int hj, int hi, double* e, double* v) {
-    *e += clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+                     double ri, int hj, int hi, double* e, double* v, double* m) {
+    double el = clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+    *e += el;
+    mag_elec(m, el);
     double vdw_e = svj * svi;
     double vdw_r = rj + ri;
     double p6 = powi6(vdw_r) / powi3(d2);
-    double vp = fmin(vdw_e * (p6 * p6 - 2.0 * p6), VDW_CUTOFF);
+    double k = vdw_e * (p6 * p6 - 2.0 * p6);
+    mag_lj(m, vdw_e, p6, k);
+    double vp = fmin(k, VDW_CUTOFF);
     if (hj && hi) {
         double d = sqrt(d2);
         double dmin = CLASH_PENALTY_FRAC * vdw_r;
         if (d < dmin) vp += CLASH_PENALTY_W * (dmin - d);
+        if (m && d < dmin * (1.0 + 1e-6)) m[2] += CLASH_PENALTY_W * dmin;
     }
     *v += vp;
 }
 
 /* vdw.rs:131-137: epsilon = sqrt(eps_i * eps_j), no clash, no electrostatics. */
-static void vdw_pair(double d2, double epsj, double epsi, double rj, double ri, double* v) {
+static void vdw_pair(double d2, double epsj, double epsi, double rj, double ri, double* v, double* m) {
     double vdw_energy = sqrt(epsi * epsj);
     double vdw_radius = ri + rj;
     double p6 = powi6(vdw_radius) / powi3(d2);
     double k = vdw_energy * (p6 * p6 - 2.0 * p6);
+    mag_lj(m, vdw_energy, p6, k);
     if (k > VDW_CUTOFF) k = VDW_CUTOFF;
     *v += k;
 }
 
 /* pydock.rs:421-428 */
 static void pydock_pair(double d2, double qj, double qi, double svj, double svi, double rj,
-                        double ri, double* e, double* v) {
-    *e += clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+                        double ri, double* e, double* v, double* m) {
+    double el = clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+    *e += el;
+    mag_elec(m, el);
     double vdw_e = svj * svi;
     double vdw_r = rj + ri;
     double p6 = powi6(vdw_r) / powi3(d2);
-    *v += fmin(vdw_e * (p6 * p6 - 2.0 * p6), VDW_CUTOFF);
+    double k = vdw_e * (p6 * p6 - 2.0 * p6);
+    mag_lj(m, vdw_e, p6, k);
+    *v += fmin(k, VDW_CUTOFF);
 }
 
 /* cpydock.rs:628-637 (powf rather than powi, epsilon = sqrt(eps_i eps_j)). */
 static void cpydock_pair(double d2, double qj, double qi, double epsj, double epsi, double rj,
-                         double ri, double* e, double* v) {
-    *e += clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+                         double ri, double* e, double* v, double* m) {
+    double el = clampd(qj * qi / d2, ELEC_MIN_CUTOFF, ELEC_MAX_CUTOFF);
+    *e += el;
+    mag_elec(m, el);
     double vdw_e = sqrt(epsj * epsi);
     double vdw_r = rj + ri;
     double p6 = pow(vdw_r, 6.0) / pow(d2, 3.0);
     double k = vdw_e * (p6 * p6 - 2.0 * p6);
+    mag_lj(m, vdw_e, p6, k);
     *v += k > VDW_CUTOFF ? VDW_CUTOFF : k;
 }
 
@@ -450,7 +473,7 @@ static void dna_grid_with(const crec* cr, const orc_mol* rec, const orc_mol* lig
     orc_transform(lig->xyz, nl, pose7, lc);
     double csp = 10.0, cox = cr->c_origin[0], coy = cr->c_origin[1], coz = cr->c_origin[2];
     int64_t ncx = cr->nc[0], ncy = cr->nc[1], ncz = cr->nc[2];
-    double e_sum = 0.0, v_sum = 0.0;
+    double e_sum = 0.0, v_sum = 0.0, mag[3] = {0.0, 0.0, 0.0};
     int64_t n_near = 0, n_cand = 0;
     for (int j = 0; j < nl; ++j) {
         double x = lc[3 * j], y = lc[3 * j + 1], z = lc[3 * j + 2];
@@ -479,7 +502,7 @@ static void dna_grid_with(const crec* cr, const orc_mol* rec, const orc_mol* lig
                             ++n_near;
                             dna_pair(d2, lig->ele[j], rec->ele[i], lig->seps[j], rec->seps[i],
                                      lig->rad[j], rec->rad[i], lig->heavy[j], rec->heavy[i],
-                                     &e_sum, &v_sum);
+                                     &e_sum, &v_sum, mag);
                         }
                     }
                 }
@@ -487,13 +510,19 @@ static void dna_grid_with(const crec* cr, const orc_mol* rec, const orc_mol* lig
         }
     }
     int64_t n_far = 0;
-    double e_field = f ? far_field_energy(f, lc, lig->ele, nl, &n_far) : 0.0;
+    double f_abs = 0.0;
+    double e_field = f ? far_field_energy(f, lc, lig->ele, nl, &n_far, &f_abs) : 0.0;
     out->e_raw = e_sum + e_field;
     out->v = v_sum;
     out->s = 0.0;
     out->n_near = n_near;
     out->n_far = n_far;
     out->n_cand = n_cand;
+    out->e_abs = mag[0];
+    out->f_abs = f_abs;
+    out->v_abs = mag[1];
+    out->c_abs = mag[2];
+    out->s_abs = 0.0;
     free(lc);
 }
 
@@ -600,7 +629,7 @@ static void nc_grid_with(int family, const nearcells* nc, const orc_mol* rec, co
         for (int i = 0; i < nr; ++i) min_rec[i] = HUGE_DIST;
         for (int j = 0; j < nl; ++j) min_lig[j] = HUGE_DIST;
     }
-    double e_near = 0.0, v_sum = 0.0;
+    double e_near = 0.0, v_sum = 0.0, mag[3] = {0.0, 0.0, 0.0};
     int64_t n_near = 0, n_cand = 0;
     for (int j = 0; j < nl; ++j) {
         double x = lc[3 * j], y = lc[3 * j + 1], z = lc[3 * j + 2];
@@ -619,22 +648,25 @@ static void nc_grid_with(int family, const nearcells* nc, const orc_mol* rec, co
             if (d2 <= VDW_DIST_CUTOFF2) {
                 ++n_near;
                 if (family == ORC_VDW)
-                    vdw_pair(d2, lig->eps[j], rec->eps[i], lig->rad[j], rec->rad[i], &v_sum);
+                    vdw_pair(d2, lig->eps[j], rec->eps[i], lig->rad[j], rec->rad[i], &v_sum, mag);
                 else if (family == ORC_PYDOCK)
                     pydock_pair(d2, lig->ele[j], rec->ele[i], lig->seps[j], rec->seps[i],
-                                lig->rad[j], rec->rad[i], &e_near, &v_sum);
+                                lig->rad[j], rec->rad[i], &e_near, &v_sum, mag);
                 else
                     cpydock_pair(d2, lig->ele[j], rec->ele[i], lig->eps[j], rec->eps[i],
-                                 lig->rad[j], rec->rad[i], &e_near, &v_sum);
+                                 lig->rad[j], rec->rad[i], &e_near, &v_sum, mag);
             }
         }
     }
     int64_t n_far = 0;
-    double e_field = 0.0;
-    if (family != ORC_VDW && f) e_field = far_field_energy(f, lc, lig->ele, nl, &n_far);
-    double s = 0.0;
+    double e_field = 0.0, f_abs = 0.0;
+    if (family != ORC_VDW && f) e_field = far_field_energy(f, lc, lig->ele, nl, &n_far, &f_abs);
+    double s = 0.0, s_abs = 0.0;
     if (family == ORC_CPYDOCK) {
-        for (int i = 0; i < nr; ++i) s += solv_term(min_rec[i], rec->asa[i], rec->des[i]);
+        for (int i = 0; i < nr; ++i) {
+            s += solv_term(min_rec[i], rec->asa[i], rec->des[i]);
+            if (min_rec[i] <= 6.4 * 6.4) s_abs += fabs(rec->des[i]);
+        }
         for (int j = 0; j < nl; ++j) s += solv_term(min_lig[j], lig->asa[j], lig->des[j]);
     }
     out->e_raw = e_near + e_field;
@@ -643,6 +675,11 @@ static void nc_grid_with(int family, const nearcells* nc, const orc_mol* rec, co
     out->n_near = n_near;
     out->n_far = n_far;
     out->n_cand = n_cand;
+    out->e_abs = mag[0];
+    out->f_abs = f_abs;
+    out->v_abs = mag[1];
+    out->c_abs = mag[2];
+    out->s_abs = s_abs;
     free(min_rec);
     free(min_lig);
     free(win);
@@ -692,6 +729,7 @@ typedef struct {
     int n_pose, tid, nthreads;
     double *oe, *ov, *os;
     int64_t *onear, *ofar;
+    double* omag;
 } batch_job;
 
 static void* batch_worker(void* arg) {
@@ -706,6 +744,14 @@ static void* batch_worker(void* arg) {
         if (b->os) b->os[k] = p.s;
         if (b->onear) b->onear[k] = p.n_near;
         if (b->ofar) b->ofar[k] = p.n_far;
+        if (b->omag) {
+            double* m = b->omag + 5 * (size_t)k;
+            m[0] = p.e_abs;
+            m[1] = p.f_abs;
+            m[2] = p.v_abs;
+            m[3] = p.c_abs;
+            m[4] = p.s_abs;
+        }
     }
     return NULL;
 }
@@ -714,6 +760,14 @@ void orc_batch_energy_grid(int family, const orc_mol* rec, const orc_mol* lig,
                            const orc_field* f, const double* poses, int n_pose,
                            double* out_e, double* out_v, double* out_s,
                            int64_t* out_near, int64_t* out_far, int nthreads) {
+    orc_batch_energy_grid_mag(family, rec, lig, f, poses, n_pose, out_e, out_v, out_s, out_near, out_far, NULL,
+                              nthreads);
+}
+
+void orc_batch_energy_grid_mag(int family, const orc_mol* rec, const orc_mol* lig,
+                               const orc_field* f, const double* poses, int n_pose,
+                               double* out_e, double* out_v, double* out_s,
+                               int64_t* out_near, int64_t* out_far, double* out_mag, int nthreads) {
     crec cr;
     nearcells nc;
     memset(&cr, 0, sizeof(cr));
@@ -726,7 +780,7 @@ void orc_batch_energy_grid(int family, const orc_mol* rec, const orc_mol* lig,
     pthread_t th[256];
     for (int t = 0; t < nthreads; ++t) {
         batch_job b = {family, rec, lig, f, &cr, &nc, poses, n_pose, t, nthreads,
-                       out_e, out_v, out_s, out_near, out_far};
+                       out_e, out_v, out_s, out_near, out_far, out_mag};
         jobs[t] = b;
     }
     for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, batch_worker, &jobs[t]);
@@ -763,19 +817,19 @@ void orc_bruteforce_parts(int family, const orc_mol* rec, const orc_mol* lig,
             ++n_near;
             if (family == ORC_DNA)
                 dna_pair(d2, lig->ele[j], rec->ele[i], lig->seps[j], rec->seps[i], lig->rad[j],
-                         rec->rad[i], lig->heavy[j], rec->heavy[i], &e, &v);
+                         rec->rad[i], lig->heavy[j], rec->heavy[i], &e, &v, NULL);
             else if (family == ORC_VDW)
-                vdw_pair(d2, lig->eps[j], rec->eps[i], lig->rad[j], rec->rad[i], &v);
+                vdw_pair(d2, lig->eps[j], rec->eps[i], lig->rad[j], rec->rad[i], &v, NULL);
             else if (family == ORC_PYDOCK)
                 pydock_pair(d2, lig->ele[j], rec->ele[i], lig->seps[j], rec->seps[i],
-                            lig->rad[j], rec->rad[i], &e, &v);
+                            lig->rad[j], rec->rad[i], &e, &v, NULL);
             else
                 cpydock_pair(d2, lig->ele[j], rec->ele[i], lig->eps[j], rec->eps[i],
-                             lig->rad[j], rec->rad[i], &e, &v);
+                             lig->rad[j], rec->rad[i], &e, &v, NULL);
         }
     }
     int64_t n_far = 0;
-    if (family != ORC_VDW && f) e += far_field_energy(f, lc, lig->ele, nl, &n_far);
+    if (family != ORC_VDW && f) e += far_field_energy(f, lc, lig->ele, nl, &n_far, NULL);
     double s = 0.0;
     if (family == ORC_CPYDOCK) {
         for (int i = 0; i < nr; ++i) s += solv_term(min_rec[i], rec->asa[i], rec->des[i]);

@@ -67,6 +67,14 @@ void orc_transform(const double* base, int nl, const double* pose7, double* out)
 typedef struct {
     double e_raw, v, s;
     int64_t n_near, n_far, n_cand;
+    /* Magnitudes for the FP32 rounding bound of a device evaluation of the
+     * same sum: sum over the pairs of |clamped electrostatic term| (e_abs),
+     * sum over charged ligand atoms of |q phi| (f_abs), sum over the pairs
+     * whose f64 LJ term is not clearly capped of |eps_ij| (p6^2 + 2 p6)
+     * (v_abs), sum over pairs within 1e-6 of the clash distance or closer of
+     * the clash weight times that distance (c_abs), and the sum of |des| over
+     * receptor atoms inside the desolvation window (s_abs). */
+    double e_abs, f_abs, v_abs, c_abs, s_abs;
 } orc_parts;
 
 /* dna.rs:834-998 */
@@ -99,6 +107,12 @@ void orc_batch_energy_grid(int family, const orc_mol* rec, const orc_mol* lig,
                            const orc_field* f, const double* poses, int n_pose,
                            double* out_e, double* out_v, double* out_s,
                            int64_t* out_near, int64_t* out_far, int nthreads);
+/* The same batch, also returning per pose the five rounding magnitudes of
+ * orc_parts (out_mag: n_pose x 5, e_abs f_abs v_abs c_abs s_abs). */
+void orc_batch_energy_grid_mag(int family, const orc_mol* rec, const orc_mol* lig,
+                               const orc_field* f, const double* poses, int n_pose,
+                               double* out_e, double* out_v, double* out_s,
+                               int64_t* out_near, int64_t* out_far, double* out_mag, int nthreads);
 
 /* FP64 brute-force re-evaluation of the same grid-split sum (no cell list):
  * every (i, j) pair, same terms, same far field.  Pins the cell enumeration. */

@@ -31,7 +31,9 @@ class Field(C.Structure):
 
 class Parts(C.Structure):
     _fields_ = [("e_raw", C.c_double), ("v", C.c_double), ("s", C.c_double),
-                ("n_near", C.c_int64), ("n_far", C.c_int64), ("n_cand", C.c_int64)]
+                ("n_near", C.c_int64), ("n_far", C.c_int64), ("n_cand", C.c_int64),
+                ("e_abs", C.c_double), ("f_abs", C.c_double), ("v_abs", C.c_double),
+                ("c_abs", C.c_double), ("s_abs", C.c_double)]
 
 
 def build():
@@ -85,6 +87,10 @@ def lib(path=None):
         L.orc_batch_energy_grid.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol),
                                             C.POINTER(Field), _d, C.c_int, _d, _d, _d,
                                             C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.c_int]
+        L.orc_batch_energy_grid_mag.restype = None
+        L.orc_batch_energy_grid_mag.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol),
+                                                C.POINTER(Field), _d, C.c_int, _d, _d, _d,
+                                                C.POINTER(C.c_int64), C.POINTER(C.c_int64), _d, C.c_int]
         L.orc_bruteforce_parts.restype = None
         L.orc_bruteforce_parts.argtypes = [C.c_int, C.POINTER(Mol), C.POINTER(Mol),
                                            C.POINTER(Field), _d, C.POINTER(Parts)]
@@ -201,21 +207,29 @@ class Oracle:
             fn(C.byref(self.rec.s), C.byref(self.lig.s), self._fp(), _p(p), C.byref(out))
         return out
 
-    def batch(self, method, poses, nthreads=1, counts=False):
-        """Batched energy_grid -> dict of arrays e, v, s (+ n_near, n_far)."""
+    def batch(self, method, poses, nthreads=1, counts=False, bound=False):
+        """Batched energy_grid -> dict of arrays e, v, s (+ n_near, n_far with
+        counts; + "bound", the FP32 rounding bound of each folded score
+        (fp32_bound), with bound)."""
         L = lib(self.lib_path)
         p = np.ascontiguousarray(poses, np.float64).reshape(-1, 7)
         n = p.shape[0]
         e, v, s = np.zeros(n), np.zeros(n), np.zeros(n)
         nn = np.zeros(n, np.int64)
         nf = np.zeros(n, np.int64)
-        L.orc_batch_energy_grid(FAMILY[method], C.byref(self.rec.s), C.byref(self.lig.s),
-                                self._fp() if method != "vdw" else None, _p(p), n, _p(e), _p(v),
-                                _p(s), _p(nn, C.c_int64) if counts else None,
-                                _p(nf, C.c_int64) if counts else None, int(nthreads))
+        mag = np.zeros((n, 5))
+        counts = counts or bound
+        L.orc_batch_energy_grid_mag(FAMILY[method], C.byref(self.rec.s), C.byref(self.lig.s),
+                                    self._fp() if method != "vdw" else None, _p(p), n, _p(e), _p(v),
+                                    _p(s), _p(nn, C.c_int64) if counts else None,
+                                    _p(nf, C.c_int64) if counts else None,
+                                    _p(mag) if bound else None, int(nthreads))
         out = {"e": e, "v": v, "s": s}
         if counts:
             out["n_near"], out["n_far"] = nn, nf
+        if bound:
+            out["mag"] = mag
+            out["bound"] = fp32_bound(method, mag, nn, nf, self.rec.s.n, self.lig.s.n)
         return out
 
     def bruteforce(self, method, pose):
@@ -283,6 +297,46 @@ class Oracle:
         out = np.zeros_like(xyz)
         lib().orc_transform(_p(xyz), xyz.shape[0], _p(p), _p(out))
         return out
+
+
+# ── FP32 rounding bound of the device's evaluation of the same sum ───────────
+U32 = 2.0 ** -24  # unit roundoff of f32
+
+
+def fp32_bound(method, mag, n_near, n_far, nr, nl):
+    """Per-pose bound on |device score - oracle score| from the device's
+    operation sequence (first order in u = 2^-24; the f64 oracle's own
+    rounding is below 1e-15 relative and neglected), with the magnitudes the
+    oracle accumulates (orc_parts):
+      electrostatics, per pair: d = (x_hi - r) + x_lo (2 roundings), d2 by a
+        product and two fmas (3), hardware reciprocal (1 ulp = 2u), two
+        products  ->  |dE_ij| <= 11u |E_ij|, plus 2^-38 per pair for the
+        fixed-point rounding (2^37 units);
+      far field: f64 from the same f32 grid values as the CPU: 2^-45 of
+        sum |q phi| for the evaluation order, 2^-38 per sampled atom;
+      LJ, per pair: sqrt(eps_i) sqrt(eps_j) from f32 inputs against the
+        CPU's epsilon (3u), r_i + r_j (u), t = (r_i + r_j)^2 / d2 (13u),
+        p6 = t^3 (41u), p6^2 - 2 p6 and the product  ->  |dV_ij| <= 89u
+        |eps_ij| (p6^2 + 2 p6) over the pairs not clearly capped (the cap is
+        exact), plus 2^-27 per pair (2^26 units);
+      clash: sqrt (1 ulp) of d2 and 0.75 (r_i + r_j)  ->  10u of the clash
+        weight times the clash distance, over the pairs at or inside it;
+      desolvation: receptor minima kept as f32 keys (u of d2, so 5u sqrt(d2)
+        <= 32u of 10 sqrt(d2) in the window) -> 32u sum |des|, plus 2^-38 per
+        atom.
+    The folds weight E by 332 / 4 and V by 1 (cpydock 0.1)."""
+    mag = np.asarray(mag, np.float64).reshape(-1, 5)
+    n_near = np.asarray(n_near, np.float64)
+    n_far = np.asarray(n_far, np.float64)
+    u = U32
+    b_e = 11 * u * mag[:, 0] + 2.0 ** -45 * mag[:, 1] + 2.0 ** -38 * (n_near + n_far)
+    b_v = 89 * u * mag[:, 2] + 10 * u * mag[:, 3] + 2.0 ** -27 * n_near
+    b_s = 32 * u * mag[:, 4] + 2.0 ** -38 * (nr + nl)
+    if method == "vdw":
+        return b_v
+    if method == "cpydock":
+        return 83.0 * b_e + 0.1 * b_v + b_s
+    return 83.0 * b_e + b_v
 
 
 def fold(method, e, v, s=0.0):

@@ -2,7 +2,9 @@
 against the oracle's f64 CPU grid values.
 
 Bar: per-pose score within 1e-5 relative of the oracle (absolute floor 1
-kcal/mol for scores near 0), each component within the same budget.
+kcal/mol for scores near 0), each component within the same budget; where the
+oracle also returns the FP32 rounding bound of the same sum (its "bound"), the
+error must lie within it too.
 """
 import sys
 
@@ -31,5 +33,8 @@ def assert_scores(method, gpu, orc, rel=REL, floor=FLOOR):
     assert (np.abs(gpu["v"] - orc["v"]) <= tol).all()
     if gpu.get("s") is not None:
         assert (np.abs(gpu["s"] - orc["s"]) <= tol).all()
+    if "bound" in orc:  # the FP32 rounding bound of the same sum (oracle.fp32_bound)
+        say(method, "max error / FP32 bound %.3f" % float((err / orc["bound"]).max()))
+        assert (err <= orc["bound"]).all(), (method, float((err / orc["bound"]).max()))
 
 

@@ -119,23 +119,24 @@ def test_1azp_cpydock(gpu):
     flags = lklight.family_flags("cpydock") | lklight.F_DESOLV
     e, v, s = ctx.score(poses, flags)
     ctx.close()
-    ref = orc.batch("cpydock", poses, nthreads=8)
+    ref = orc.batch("cpydock", poses, nthreads=8, bound=True)
     g = lklight.fold_scores("cpydock", e, v, s)
     o = lklight.fold_scores("cpydock", ref["e"], ref["v"], ref["s"])
     err = np.abs(g - o) / np.maximum(np.abs(o), FLOOR)
     # Some poses of this complex score near 0 as the difference of terms of
-    # +-40..90 (E*83, 0.1 V, S).  The device's f32 pair arithmetic (the
-    # reference's GPU kernel computes in f32 as well) then leaves an error of
-    # ~2e-7 of those terms, which can exceed 1e-5 of a score below 1.  The bar
-    # is 1e-5 of max(|score|, 1) plus 1e-7 of the sum of the terms' magnitudes
-    # (the rounding bound of a sum evaluated in f32).
-    mag = np.abs(ref["e"]) * 83.0 + 0.1 * np.abs(ref["v"]) + np.abs(ref["s"])
-    tol = REL * np.maximum(np.abs(o), FLOOR) + 1e-7 * mag
-    say("cpydock poses", poses.shape[0], "max rel err %.3e, max err / bar %.3f"
-        % (float(err.max()), float((np.abs(g - o) / tol).max())))
-    assert (np.abs(g - o) <= tol).all(), int(np.argmax(np.abs(g - o) / tol))
-    well = np.abs(o) >= 0.05 * mag  # scores that are not a near-cancellation
-    assert (err[well] <= REL).all(), float(err[well].max())
+    # +-40..90 (E*83, 0.1 V, S).  The device evaluates the pair terms in f32
+    # (as the reference's GPU kernel does), and the rounding bound of that
+    # evaluation (oracle fp32_bound: per-pair relative error from the
+    # operation sequence times the magnitudes of the terms) can then exceed
+    # 1e-5 of a score below 1.  Every pose is held to that derived bound, and
+    # every pose whose bound is below the 1e-5 bar to the bar.
+    bound = ref["bound"]
+    say("cpydock poses", poses.shape[0], "max rel err %.3e, max error / FP32 bound %.3f"
+        % (float(err.max()), float((np.abs(g - o) / bound).max())))
+    assert (np.abs(g - o) <= bound).all(), int(np.argmax(np.abs(g - o) / bound))
+    bar = bound <= REL * np.maximum(np.abs(o), FLOOR)
+    assert (err[bar] <= REL).all(), float(err[bar].max())
+    say("cpydock poses held to the 1e-5 bar: %d of %d" % (int(bar.sum()), poses.shape[0]))
     for k, t in enumerate(A.CPYDOCK_POSES):
         ex = orc.cpydock_exact(A.pose7(t))
         assert abs(g[k] - ex) < A.CPYDOCK_GRID_EXACT, (t, g[k], ex)

@@ -497,7 +497,8 @@ def test_config2_full_size(c2, method):
     full = run(ctx, poses, flags)
     sub = poses[::40]
     say("c2 oracle subset", sub.shape[0])
-    assert_scores(method, {"e": full["e"][::40], "v": full["v"][::40]}, orc.batch(method, sub, nthreads=8))
+    assert_scores(method, {"e": full["e"][::40], "v": full["v"][::40]}, orc.batch(method, sub, nthreads=8,
+                                                                               bound=True))
     # batch split and repeat invariance at full size
     again = run(ctx, poses[600:], flags)
     assert np.array_equal(again["e"], full["e"][600:]) and np.array_equal(again["v"], full["v"][600:])

@@ -261,3 +261,50 @@ def test_screen_candidates_oracle_pins():
     assert (cosang > (1 - 2 * rs) / (1 + 2 * rs)).all()
     assert np.array_equal(O.screen_candidates(base, C, tr, rs, 77), c.reshape(-1, 7))
     assert not np.array_equal(O.screen_candidates(base, C, tr, rs, 78), c.reshape(-1, 7))
+
+
+def test_fp32_bound_magnitudes_and_soundness():
+    """The oracle's rounding magnitudes (orc_parts e_abs, v_abs) equal the
+    same sums over all pairs within 10 A computed directly in numpy (dna on
+    the small complex), and the bound they give covers an f32 evaluation of
+    the pair terms in the device's operation order (hi/lo coordinate split,
+    d2 from a product and two fmas, f32 reciprocal and products): the f32
+    electrostatic sum stays within 11u sum|E_ij| and the f32 LJ sum within
+    89u of its magnitude on every pose."""
+    cx = synth.make_config("small")
+    orc = O.Oracle(cx)
+    poses = synth.make_poses(12, cx.meta["rec_axes"], 21)
+    r = orc.batch("dna", poses, bound=True)
+    rx = np.asarray(cx.rec.xyz, float)
+    f32 = np.float32
+    u = 2.0 ** -24
+    for k, pose in enumerate(poses):
+        lx = orc.transform(pose)
+        d = lx[:, None, :] - rx[None, :, :]
+        d2 = (d * d).sum(-1)
+        near = d2 <= 100.0
+        qq = np.outer(cx.lig.ele, cx.rec.ele)[near]
+        e64 = np.clip(qq / d2[near], -4 / 332, 4 / 332)
+        assert np.isclose(np.abs(e64).sum(), r["mag"][k, 0], rtol=1e-12, atol=0)
+        eps = np.outer(cx.lig.seps, cx.rec.seps)[near]
+        rr = np.add.outer(cx.lig.rad, cx.rec.rad)[near]
+        p6 = rr ** 6 / d2[near] ** 3
+        lj = eps * (p6 * p6 - 2 * p6)
+        keep = lj <= 2.0
+        assert np.isclose((np.abs(eps) * (p6 * p6 + 2 * p6))[keep].sum(), r["mag"][k, 2], rtol=1e-12, atol=0)
+        # f32 in the device's order (numpy's f32 division is correctly
+        # rounded; the hardware reciprocal's extra ulp is inside the bound)
+        xh = lx.astype(f32)
+        xl = (lx - xh.astype(float)).astype(f32)
+        dd = (xh[:, None, :] - rx.astype(f32)[None, :, :]) + xl[:, None, :]
+        d2f = (dd[..., 0] * dd[..., 0] + dd[..., 1] * dd[..., 1] + dd[..., 2] * dd[..., 2])[near]
+        inv = f32(1.0) / d2f
+        qf = np.outer(cx.lig.ele.astype(f32), cx.rec.ele.astype(f32))[near]
+        ef = np.clip((qf * inv).astype(float), -4 / 332, 4 / 332)
+        assert abs(ef.sum() - e64.sum()) <= 11 * u * np.abs(e64).sum()
+        rrf = np.add.outer(cx.lig.rad.astype(f32), cx.rec.rad.astype(f32))[near]
+        t = (rrf * rrf) * inv
+        p6f = t * t * t
+        ljf = (np.outer(cx.lig.seps.astype(f32), cx.rec.seps.astype(f32))[near] * (p6f * p6f - f32(2) * p6f))
+        cap = np.minimum(ljf.astype(float), 1.0)
+        assert abs(cap.sum() - np.minimum(lj, 1.0).sum()) <= 89 * u * r["mag"][k, 2] + 1e-12
