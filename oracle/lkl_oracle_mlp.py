@@ -130,3 +130,65 @@ def adam(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     mh = m / (1 - b1 ** step)
     vh = v / (1 - b2 ** step)
     return p - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+# ── spread of valid fp32 evaluations (tolerances of the device checks) ────────
+# The kernels take the same bf16 operands as forward(round=True) but
+# accumulate in fp32 (MFMA tiles, split-K partials, fp32 epilogues), in an
+# order of their own; a hidden activation then rounds to a neighbouring bf16
+# value now and then, and the gradients inherit those differences through the
+# bf16 dZ operands.  A worst-case bound of that chain grows by the layers'
+# absolute row sums (tens per layer) and is vacuous, so the checks compare the
+# device with the spread of other valid fp32 evaluations of the same network:
+# the same operands and bf16 roundings (activations, dZ operands) with fp32
+# matrix products whose summation order is permuted (a random permutation of
+# the reduced dimension per evaluation).
+
+
+def _mm32(a, b, perm):
+    """fp32 product with the reduced dimension permuted."""
+    return np.asarray(a, np.float32)[:, perm] @ np.asarray(b, np.float32)[perm, :]
+
+
+def forward_f32(p, x, seed):
+    """forward(round=True) with fp32 products in a permuted order: y, cache."""
+    rng = np.random.default_rng(seed)
+    P = split(p)
+    X = bf16_to_f64(x).astype(np.float32)
+    W = {k: round_bf16(P[k]).astype(np.float32) for k in ("W1", "W2", "W3", "W4")}
+    act = lambda z: round_bf16(np.maximum(z, 0.0)).astype(np.float32)
+    h1 = act(_mm32(X, W["W1"], rng.permutation(NF)) + P["b1"].astype(np.float32))
+    h2 = act(_mm32(h1, W["W2"], rng.permutation(D)) + P["b2"].astype(np.float32))
+    h3 = act(_mm32(h2, W["W3"], rng.permutation(D)) + P["b3"].astype(np.float32))
+    pk = rng.permutation(D)
+    y = (h3[:, pk] @ W["W4"][pk] + np.float32(P["b4"][0])).astype(np.float64)
+    return y, {"X": X, "W": W, "h1": h1, "h2": h2, "h3": h3}
+
+
+def loss_grad_f32(p, x, t, seed):
+    """loss_grad(round=True) evaluated like the kernels, fp32 products in a
+    permuted order, dZ rounded to bf16 where the kernels use it as a GEMM
+    operand.  Returns (loss, flat gradient f64, y)."""
+    y, c = forward_f32(p, x, seed)
+    rng = np.random.default_rng(seed + 1)
+    t = np.asarray(t, np.float64)
+    B = y.shape[0]
+    dy = ((y.astype(np.float32) - t.astype(np.float32)) / np.float32(B)).astype(np.float32)
+    loss = float(0.5 * np.mean((y - t) ** 2))
+    W = c["W"]
+    bf = lambda a: round_bf16(a).astype(np.float32)
+    g = {}
+    g["W4"] = (c["h3"].T @ dy).astype(np.float64)
+    g["b4"] = np.array([float(dy.sum(dtype=np.float32))])
+    dz = np.outer(dy, W["W4"]).astype(np.float32) * (c["h3"] > 0)
+    hs = {3: c["h2"], 2: c["h1"], 1: c["X"]}
+    for l in (3, 2, 1):
+        dzb = bf(dz)
+        g["W%d" % l] = _mm32(hs[l].T, dzb, rng.permutation(B)).astype(np.float64)
+        g["b%d" % l] = dz.sum(0, dtype=np.float32).astype(np.float64)
+        if l > 1:
+            dz = _mm32(dzb, W["W%d" % l].T, rng.permutation(D)) * (hs[l] > 0)
+    flat = np.zeros(NPARAM, np.float64)
+    for k, (o, n) in offsets().items():
+        flat[o:o + n] = np.asarray(g[k]).ravel()
+    return loss, flat, y

@@ -150,17 +150,24 @@ def feats(c1):
 
 @pytest.mark.gpu
 def test_mlp_forward_matches_oracle(gpu, feats):
-    """fp32 accumulation vs exact f64 sums with the same bf16 operands: hidden
-    activations may round to neighbouring bf16 values, so y agrees to a small
-    fraction of its spread (tolerance 1% of max |y|)."""
+    """Device forward (bf16 operands, fp32 MFMA accumulation) against the f64
+    forward of the same bf16 operands, held to the spread of other valid fp32
+    evaluations (oracle forward_f32: the same operands and bf16 activation
+    roundings with fp32 products summed in 6 permuted orders): the largest
+    error within 4x the largest error of those evaluations, the RMS within 3x
+    theirs."""
     _, x = feats
     net = S.MLP(0)
     p = _params(1)
     net.params = p
     y = net.forward(x)
     yr, _ = M.forward(p, x, round=True)
-    assert np.abs(y - yr).max() <= 1e-2 * np.abs(yr).max(), (np.abs(y - yr).max(), np.abs(yr).max())
-    assert np.corrcoef(y, yr)[0, 1] > 0.9999
+    sims = np.array([M.forward_f32(p, x, s)[0] - yr for s in range(6)])
+    err = y - yr
+    print("[gpu-test] forward max err %.3e (fp32 evaluations %.3e), rms %.3e (%.3e)"
+          % (np.abs(err).max(), np.abs(sims).max(), np.sqrt((err ** 2).mean()), np.sqrt((sims ** 2).mean())))
+    assert np.abs(err).max() <= 4.0 * np.abs(sims).max()
+    assert np.sqrt((err ** 2).mean()) <= 3.0 * np.sqrt((sims ** 2).mean())
     # the create() initialisation is also a valid network
     net2 = S.MLP(123)
     assert np.isfinite(net2.forward(x)).all()
@@ -170,8 +177,11 @@ def test_mlp_forward_matches_oracle(gpu, feats):
 
 @pytest.mark.gpu
 def test_mlp_gradients_match_oracle(gpu, feats):
-    """Per-tensor relative error of the gradient (bf16 dz operands in the
-    kernels, exact f64 in the oracle): norm error <= 3%, cosine >= 0.999."""
+    """Per-tensor gradient error of the device (bf16 dZ GEMM operands, fp32
+    accumulation, split-K partials) against the f64 gradient of the same bf16
+    operands, held to 4x the largest error of 4 other valid fp32 evaluations
+    (oracle loss_grad_f32: permuted fp32 summation orders, the kernels' bf16
+    roundings).  The loss within the same spread."""
     _, x = feats
     rng = np.random.default_rng(2)
     t = rng.normal(size=x.shape[0]).astype(np.float32)
@@ -181,15 +191,15 @@ def test_mlp_gradients_match_oracle(gpu, feats):
     loss = net.grad(x, t)
     g = net.grads()
     lr, gr, _ = M.loss_grad(p, x, t, round=True)
-    assert abs(loss - lr) <= 1e-2 * lr
+    sims = [M.loss_grad_f32(p, x, t, s) for s in range(4)]
+    assert abs(loss - lr) <= 4.0 * max(abs(sl - lr) for sl, _, _ in sims) + 1e-7 * lr
     for k, (o, n) in M.offsets().items():
         a, b = g[o:o + n].astype(np.float64), gr[o:o + n]
-        if k == "b4":
-            assert abs(a[0] - b[0]) <= 1e-2 * max(abs(b[0]), 1e-3)
-            continue
-        rel = np.linalg.norm(a - b) / np.linalg.norm(b)
-        cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
-        assert rel <= 3e-2 and cos >= 0.999, (k, rel, cos)
+        e = np.linalg.norm(a - b)
+        es = max(np.linalg.norm(sg[o:o + n] - b) for _, sg, _ in sims)
+        print("[gpu-test] %s: error norm %.3e, fp32 evaluations %.3e (relative %.2e)"
+              % (k, e, es, e / np.linalg.norm(b)))
+        assert e <= 4.0 * es + 1e-7 * np.linalg.norm(b), (k, e, es)
     net.close()
 
 

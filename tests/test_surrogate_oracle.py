@@ -191,6 +191,27 @@ def test_forward_rounding_modes_agree_roughly():
     assert np.abs(yb - y64).max() <= 0.05 * np.abs(y64).max() + 1e-6
 
 
+def test_fp32_evaluations_are_the_same_network():
+    """The permuted-order fp32 evaluations (the tolerance reference of the
+    device checks) compute the same function as the f64 oracle: outputs and
+    every gradient tensor within 4e-3 relative (the fp32 evaluations, like the
+    kernels, round the dZ operands to bf16: 2^-9 each), and different
+    summation orders give (slightly) different bits, as intended."""
+    p, x, t = _tiny_net(6)
+    xb = M.features_bits(np.expm1(x))
+    y64, _ = M.forward(p, xb, round=True)
+    ya, _ = M.forward_f32(p, xb, 1)
+    yb, _ = M.forward_f32(p, xb, 2)
+    assert np.abs(ya - y64).max() <= 1e-3 * np.abs(y64).max()
+    _, g64, _ = M.loss_grad(p, xb, t, round=True)
+    _, ga, _ = M.loss_grad_f32(p, xb, t, 1)
+    _, gb, _ = M.loss_grad_f32(p, xb, t, 2)
+    for k, (o, n) in M.offsets().items():
+        ref = np.linalg.norm(g64[o:o + n])
+        assert np.linalg.norm(ga[o:o + n] - g64[o:o + n]) <= 4e-3 * ref + 1e-12, k
+    assert not np.array_equal(ga, gb)
+
+
 @pytest.mark.parametrize("h", [0.5, 1.0])
 def test_grid_node_counts_are_pair_counts_at_the_node(h):
     """A node's tabulated counts equal the exact pair histogram of a single
