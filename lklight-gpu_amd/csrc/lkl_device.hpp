@@ -80,6 +80,8 @@ struct DevComplex {
     const unsigned* nidx32;
     // the sorted receptor records interleaved for the list walk: rrec[2p] =
     // rxyzq[p], rrec[2p + 1] = rpar[p] (one 32-byte record per atom, nr + 1)
+    // except that the radius carries the sign bit where the atom is excluded
+    // from desolvation (rflag[p] != 0, and the padding record)
     const float4* rrec;
     // position in nidx of a block of LIST_PAD entries holding nr, past the
     // last list: a lane whose list has ended loads its indices from there

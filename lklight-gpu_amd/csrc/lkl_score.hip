@@ -108,7 +108,10 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
     const float dz = (zh - r.z) + zl;
     const float d2 = dx * dx + dy * dy + dz * dz;
     if (DES != DES_NONE && d2 <= SOLV_HI && !lf) {
-        if (C.rflag[p] == 0) {
+        // in the interleaved records the receptor atom's desolvation flag is
+        // the sign bit of its radius (set: excluded), so the walk needs no
+        // extra gather for it
+        if (PRE ? (__float_as_uint(rp_pre.y) >> 31) == 0u : C.rflag[p] == 0) {
             const double d2e = exact_d2(x64, y64, z64, r);
             minL = fmin(minL, d2e);
             atomicMin(&minR[p], solv_key(d2e));
@@ -122,7 +125,7 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
             e += min(max(__float2int_rn(ae), -(int)ES_CAP_I), (int)ES_CAP_I);
         }
         const float svs = svjs * rp.x;
-        const float rr = rj + rp.y;
+        const float rr = rj + fabsf(rp.y);
         const float t = (rr * rr) * inv;  // (r_i + r_j)^2 / d^2
         const float p6 = t * t * t;
         float vp = svs * (p6 * p6 - 2.0f * p6);
@@ -1461,9 +1464,13 @@ static int build_receptor(lkl_ctx* c) {
     if (dev_upload(&c->d_rpar, par.data(), (size_t)nr + 1, c->stream)) return -1;
     {
         std::vector<float4> rec(2 * ((size_t)nr + 1));
+        // (x, y, z, q) then (svdw, radius, heavy, 0); a receptor atom excluded
+        // from desolvation (and the padding record) carries its radius with
+        // the sign bit set, which the pair terms take the magnitude of
         for (size_t p = 0; p <= (size_t)nr; ++p) {
             rec[2 * p] = xyzq[p];
             rec[2 * p + 1] = par[p];
+            if (flag[p]) rec[2 * p + 1].y = -std::fabs(par[p].y);
         }
         if (dev_upload(&c->d_rrec, rec.data(), rec.size(), c->stream)) return -1;
     }
