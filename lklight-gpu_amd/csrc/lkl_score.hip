@@ -708,11 +708,28 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
     __syncthreads();  // the LDS arrays may be reused by the caller
 }
 
+// Cross-tile exchange without cache maintenance.  Tiles of one pose meet only
+// through device-scope atomics (read-modify-writes, and loads and stores of
+// the atomic kind), which are coherent across the chip's L2 caches without
+// the L2 write-back and invalidation that a device-scope fence costs.  What a
+// fence would add here is ordering: vm_drain() waits until every global memory
+// operation this wave issued is performed (s_waitcnt vmcnt(0); a workgroup
+// barrier does not wait for them), so a tile's merges and partial sums are in
+// place before the wave that counts the tile in issues that count.  The tile
+// that sees the count complete reads them with device-scope atomic loads,
+// issued after its count returned.
+__device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // One workgroup per (pose, chunk of `chunk` ligand atoms) tile.  out[2k] = E_raw,
 // out[2k+1] = V, outS[k] = S.  With several chunks per pose the tiles add their
 // fixed-point partials into acc[3k..3k+2] (integer atomics: exact, order-free);
 // the last tile of a pose (counter cnt[k]) writes the result and re-zeroes acc
-// and cnt for the next launch.  Desolvation needs the whole ligand in one tile.
+// and cnt for the next launch.  Desolvation with global minima (DES_GLOBAL)
+// needs the whole ligand in one tile.  With minima in LDS a pose may be split:
+// each tile's ligand atoms take their own minima whole, each tile merges its
+// receptor minima into the pose's row of gmin (global atomic minima, only the
+// entries it lowered), and the pose's last tile sums the receptor terms from
+// gmin and restores its initial keys (gmin holds them between launches).
 // PERATOM also writes each ligand atom's (E, V) (sorted order) to pa_e/pa_v.
 #ifndef LKL_SCORE_ATTR
 #define LKL_SCORE_ATTR
@@ -728,6 +745,8 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
              int chunk, int nchunks) {
     extern __shared__ __attribute__((aligned(16))) unsigned lds_min[];
     __shared__ long long red[3][16];
+    __shared__ unsigned last_tile;
+    const bool split = DES == DES_LDS && nchunks > 1;
     const unsigned total = (unsigned)N * (unsigned)nchunks;
     if (blockIdx.x >= total) return;
     const unsigned L = xcd_tile(blockIdx.x, total);
@@ -744,7 +763,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
     if (DES != DES_NONE) {
         const unsigned init = __float_as_uint((float)MIN_INIT);
         for (int i = tid; i < C.nr; i += bs) minR[i] = init;
-        if (DES == DES_GLOBAL) __threadfence();
+        if (DES == DES_GLOBAL) vm_drain();  // the stores land before any wave's atomics
         __syncthreads();
     }
 
@@ -796,8 +815,17 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         v += va;
         if (DES != DES_NONE) s += to_fix(solv_term(minL, C.lasa[j], C.ldes[j]), E_SCALE);
     }
-    if (DES != DES_NONE) {
-        if (DES == DES_GLOBAL) __threadfence();
+    if (split) {
+        __syncthreads();
+        unsigned* __restrict__ g = gmin + (size_t)pose * (size_t)C.nr;
+        const unsigned init = __float_as_uint((float)MIN_INIT);
+        for (int p = tid; p < C.nr; p += bs) {
+            const unsigned k = minR[p];
+            if (k < init) atomicMin(&g[p], k);
+        }
+        vm_drain();  // this wave's merges are performed before the tile is counted
+    } else if (DES != DES_NONE) {
+        if (DES == DES_GLOBAL) vm_drain();  // the atomics land before any wave reads the minima
         __syncthreads();
         for (int p = tid; p < C.nr; p += bs) {
             unsigned k;
@@ -818,6 +846,47 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         red[2][w] = s;
     }
     __syncthreads();
+    if (split) {
+        if (tid == 0) {
+            long long te = 0, tv = 0, ts = 0;
+            for (int k = 0; k < nw; ++k) {
+                te += red[0][k];
+                tv += red[1][k];
+                ts += red[2][k];
+            }
+            unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
+            atomicAdd(&a[0], (unsigned long long)te);
+            atomicAdd(&a[1], (unsigned long long)tv);
+            atomicAdd(&a[2], (unsigned long long)ts);
+            vm_drain();
+            last_tile = atomicAdd(&cnt[pose], 1u) == (unsigned)nchunks - 1u ? 1u : 0u;
+        }
+        __syncthreads();
+        if (!last_tile) return;
+        unsigned* __restrict__ g = gmin + (size_t)pose * (size_t)C.nr;
+        const unsigned init = __float_as_uint((float)MIN_INIT);
+        long long sr = 0;
+        for (int p = tid; p < C.nr; p += bs) {
+            const unsigned k = __hip_atomic_load(&g[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (k != init) __hip_atomic_store(&g[p], init, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            sr += to_fix(solv_term((double)__uint_as_float(k), C.rasa[p], C.rdes[p]), E_SCALE);
+        }
+        sr = wave_sum(sr);
+        if ((tid & 63) == 0) red[2][w] = sr;
+        __syncthreads();
+        if (tid != 0) return;
+        long long tsr = 0;
+        for (int k = 0; k < nw; ++k) tsr += red[2][k];
+        unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
+        const long long te = (long long)atomicExch(&a[0], 0ull);
+        const long long tv = (long long)atomicExch(&a[1], 0ull);
+        const long long ts = (long long)atomicExch(&a[2], 0ull) + tsr;
+        atomicExch(&cnt[pose], 0u);
+        out[2 * (size_t)pose] = (double)te / E_SCALE;
+        out[2 * (size_t)pose + 1] = (double)tv / V_SCALE;
+        outS[pose] = (double)ts / E_SCALE;
+        return;
+    }
     if (tid != 0) return;
     long long te = 0, tv = 0, ts = 0;
     for (int k = 0; k < nw; ++k) {
@@ -829,9 +898,8 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
         unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
         atomicAdd(&a[0], (unsigned long long)te);
         atomicAdd(&a[1], (unsigned long long)tv);
-        __threadfence();
+        vm_drain();
         if (atomicAdd(&cnt[pose], 1u) != (unsigned)nchunks - 1u) return;
-        __threadfence();
         te = (long long)atomicExch(&a[0], 0ull);
         tv = (long long)atomicExch(&a[1], 0ull);
         atomicExch(&cnt[pose], 0u);
@@ -938,9 +1006,8 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
         unsigned long long* a = (unsigned long long*)acc + 3 * (size_t)pose;
         atomicAdd(&a[0], (unsigned long long)te);
         atomicAdd(&a[1], (unsigned long long)tv);
-        __threadfence();
+        vm_drain();  // as in score_kernel: partials performed before the count
         if (atomicAdd(&cnt[pose], 1u) != (unsigned)P.nchunks - 1u) return;
-        __threadfence();
         te = (long long)atomicExch(&a[0], 0ull);
         tv = (long long)atomicExch(&a[1], 0ull);
         atomicExch(&cnt[pose], 0u);
@@ -1203,6 +1270,8 @@ struct lkl_ctx {
     size_t cap_outS = 0;
     unsigned* d_gmin = nullptr;
     size_t cap_gmin = 0;
+    unsigned* d_smin = nullptr;  // receptor minima of split desolvation poses (held at the initial key)
+    size_t cap_smin = 0;
     float *d_pa_e = nullptr, *d_pa_v = nullptr;
     size_t cap_pa = 0, cap_pa_v = 0;
     double* d_xyz = nullptr;
@@ -1677,7 +1746,7 @@ static void ctx_free(lkl_ctx* c) {
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
                     c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder, c->d_nidx32,
-                    c->d_nidx32_if};
+                    c->d_nidx32_if, c->d_smin};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
@@ -1811,9 +1880,39 @@ __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, size_t n) {
     if (i < n) p[i] = 0ull;
 }
 
+__global__ void fill_u32_kernel(unsigned* __restrict__ p, size_t n, unsigned v) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Desolvation minima of one pose fit in LDS beside the kernel's own arrays.
 static bool des_in_lds(const lkl_ctx* c) {
     return (size_t)c->nr * sizeof(unsigned) + 1024 <= c->lds_max && !getenv("LKL_FORCE_GLOBAL_MIN");
+}
+
+// Largest global receptor-minima buffer a split desolvation launch may use
+// (N x nr keys); larger launches keep one tile per pose, which balances well
+// at that size anyway.
+constexpr size_t SPLIT_MIN_BYTES = (size_t)64 << 20;
+
+// The split launches' receptor minima: every key at the initial distance's
+// key when allocated (the last tile of each pose restores them after use).
+static int reserve_split_min(lkl_ctx* c, size_t n) {
+    if (n <= c->cap_smin && c->d_smin) return 0;
+    if (c->d_smin) { (void)hipFree(c->d_smin); c->d_smin = nullptr; }
+    c->cap_smin = 0;
+    HIPCK("alloc", hipMalloc((void**)&c->d_smin, sizeof(unsigned) * n));
+    const float init_f = (float)MIN_INIT;
+    unsigned init;
+    std::memcpy(&init, &init_f, sizeof init);
+    // on the context's stream and waited for: the first launch may be
+    // enqueued on another stream, which must not overtake the fill
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->d_smin, n,
+                       init);
+    HIPCK("alloc", hipGetLastError());
+    HIPCK("sync", hipStreamSynchronize(c->stream));
+    c->cap_smin = n;
+    return 0;
 }
 
 // Enqueue one score launch; all pointers are device pointers.
@@ -1828,6 +1927,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         C.lstride = c->nl;
     }
     int des = DES_NONE;
+    int dchunk = c->nl;  // ligand atoms per desolvation tile
     size_t shm = 0;
     // work-ordered near field: tile-local lists of (chunk x 3) bytes in LDS
     const bool want_sort = C.wgrid != nullptr && !peratom;
@@ -1842,10 +1942,15 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
             des = DES_GLOBAL;
             if (!ext && dev_reserve(&c->d_gmin, &c->cap_gmin, (size_t)N * c->nr)) return -1;
         }
-        // desolvation tiles hold the whole ligand: order the near field only
-        // if its arrays fit (8 bytes of alignment after the receptor minima)
-        const size_t tl = tile_lds((size_t)c->nl, list) + 8;
-        if (want_sort && c->nl <= WORK_TILE_MAX && shm + tl + 1024 <= c->lds_max) shm += tl;
+        // a tile holds the whole ligand, or (minima in LDS, a ligand tile
+        // set, a launch small enough for the global minima) one chunk of it;
+        // the near field is ordered if the tile's arrays fit (8 bytes of
+        // alignment after the receptor minima)
+        if (des == DES_LDS && !ext && !peratom && c->chunk > 0 && c->chunk < c->nl &&
+            (size_t)N * c->nr * sizeof(unsigned) <= SPLIT_MIN_BYTES)
+            dchunk = c->chunk;
+        const size_t tl = tile_lds((size_t)dchunk, list) + 8;
+        if (want_sort && dchunk <= WORK_TILE_MAX && shm + tl + 1024 <= c->lds_max) shm += tl;
         else C.wgrid = nullptr;
     }
     if (peratom && dev_reserve(&c->d_pa_e, &c->cap_pa, (size_t)N * c->nl)) return -1;
@@ -1856,7 +1961,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     else if (des == DES_GLOBAL)
         fn = list ? pick3<true, DES_GLOBAL, false>(flags) : pick3<false, DES_GLOBAL, false>(flags);
     else fn = list ? pick3<true, DES_NONE, false>(flags) : pick3<false, DES_NONE, false>(flags);
-    int chunk = (c->chunk > 0 && des == DES_NONE) ? std::min(c->chunk, c->nl) : c->nl;
+    int chunk = des != DES_NONE ? dchunk : c->chunk > 0 ? std::min(c->chunk, c->nl) : c->nl;
     if (des == DES_NONE && want_sort) {
         const int tmax = (int)std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (list ? 10 : 3));
         if (chunk > tmax) chunk = tmax;  // LDS bound of the tile's arrays
@@ -1873,7 +1978,9 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     const int nchunks = (c->nl + chunk - 1) / chunk;
     if ((size_t)N * nchunks >= ((size_t)1 << 31)) { set_err("too many tiles (N=%d)", N); return -1; }
     if (!ext && nchunks > 1 && reserve_acc(c, (size_t)N)) return -1;
-    unsigned* gmin = ext ? ext->gmin : c->d_gmin;
+    const bool split = des == DES_LDS && nchunks > 1;
+    if (split && reserve_split_min(c, (size_t)N * c->nr)) return -1;
+    unsigned* gmin = ext ? ext->gmin : split ? c->d_smin : c->d_gmin;
     long long* acc = ext ? ext->acc : c->d_acc;
     unsigned* cnt = ext ? ext->cnt : c->d_cnt;
     if (fail_at("launch")) { set_err("launch_score stage=launch: injected failure"); return -1; }
@@ -2416,7 +2523,11 @@ LKL_API int lkl_ctx_autotune(lkl_ctx* c, const double* poses, int N, unsigned fl
                 for (int mult : k_chunk_mult) {
                     const int chunk = mult ? block * mult : 0;
                     if (mult && chunk >= c->nl) continue;  // same as the whole-ligand tile
-                    if ((flags & F_DESOLV) && mult) continue;
+                    // desolvation tiles split a pose only with minima in LDS
+                    // and a launch within the global minima's bound
+                    if ((flags & F_DESOLV) && mult &&
+                        !(des_in_lds(c) && (size_t)N * c->nr * sizeof(unsigned) <= SPLIT_MIN_BYTES))
+                        continue;
                     c->block = block;
                     c->chunk = chunk;
                     double* dS = (flags & F_DESOLV) ? c->d_outS : nullptr;

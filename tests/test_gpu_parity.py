@@ -554,4 +554,14 @@ def test_config3_cpydock(gpu):
     assert_scores("cpydock", {k: full[k][::25] for k in ("e", "v", "s")},
                   O.Oracle(cx).batch("cpydock", sub, nthreads=8))
     assert np.abs(full["s"]).max() > 0
+    # poses split over ligand tiles (receptor minima merged in global memory
+    # by the tiles, summed by each pose's last tile) give the same bits; the
+    # second call of each shape, with fewer poses, finds the minima restored
+    for block, chunk in ((512, 1024), (1024, 1024), (256, 512)):
+        ctx.set_tiles(7.5, block, chunk, cell_x=1.25)
+        say(f"c3 split tiles: block {block} chunk {chunk}")
+        for n in (400, 150):
+            r = run(ctx, poses[:n], flags)
+            for k in ("e", "v", "s"):
+                assert np.array_equal(r[k], full[k][:n]), (block, chunk, n, k)
     ctx.close()
