@@ -167,8 +167,16 @@ __device__ __forceinline__ double exact_d2(double x, double y, double z, float4 
 // Branch-free: outside the grid (or for NaN input) the corner reads go to cell
 // 0 and the result is selected to 0, so the loads of several samples can be
 // issued together.  With a power-of-two spacing the division is a
-// multiplication by the exact reciprocal (the same IEEE result).
-__device__ __forceinline__ double sample_phi(const DevComplex& C, double x, double y, double z) {
+// multiplication by the exact reciprocal (the same IEEE result).  phi_fetch
+// issues a sample's corner reads and phi_lerp interpolates them, so a caller
+// can put other work between the two while the reads are in flight.
+struct PhiTap {
+    float c[8];       // corners 000, 100, 010, 110, 001, 101, 011, 111
+    float tx, ty, tz;  // offsets in the cell, rounded to f32
+    bool in;
+};
+
+__device__ __forceinline__ void phi_fetch(const DevComplex& C, double x, double y, double z, PhiTap& T) {
 #pragma clang fp contract(off)
     double fx, fy, fz;
     if (C.sp_pow2) {
@@ -184,24 +192,31 @@ __device__ __forceinline__ double sample_phi(const DevComplex& C, double x, doub
     const bool in = ix >= 0.0 && iy >= 0.0 && iz >= 0.0 && ix + 1.0 < (double)C.nx &&
                     iy + 1.0 < (double)C.ny && iz + 1.0 < (double)C.nz;
     const size_t jx = in ? (size_t)ix : 0, jy = in ? (size_t)iy : 0, jz = in ? (size_t)iz : 0;
-    const double tx = in ? (double)(float)(fx - ix) : 0.0;
-    const double ty = in ? (double)(float)(fy - iy) : 0.0;
-    const double tz = in ? (double)(float)(fz - iz) : 0.0;
-    double c000, c100, c010, c110, c001, c101, c011, c111;
+    T.in = in;
+    T.tx = in ? (float)(fx - ix) : 0.0f;
+    T.ty = in ? (float)(fy - iy) : 0.0f;
+    T.tz = in ? (float)(fz - iz) : 0.0f;
     if (C.phi8) {
         const size_t c = (jz * (size_t)(C.ny - 1) + jy) * (size_t)(C.nx - 1) + jx;
         const float4 lo = C.phi8[2 * c], hi = C.phi8[2 * c + 1];
-        c000 = lo.x; c100 = lo.y; c010 = lo.z; c110 = lo.w;
-        c001 = hi.x; c101 = hi.y; c011 = hi.z; c111 = hi.w;
+        T.c[0] = lo.x; T.c[1] = lo.y; T.c[2] = lo.z; T.c[3] = lo.w;
+        T.c[4] = hi.x; T.c[5] = hi.y; T.c[6] = hi.z; T.c[7] = hi.w;
     } else {
         const size_t sy = (size_t)C.nx, sz = (size_t)C.nx * (size_t)C.ny;
         const size_t b = jz * sz + jy * sy + jx;
         const float* __restrict__ g = C.phi;
-        c000 = g[b]; c100 = g[b + 1];
-        c010 = g[b + sy]; c110 = g[b + sy + 1];
-        c001 = g[b + sz]; c101 = g[b + sz + 1];
-        c011 = g[b + sz + sy]; c111 = g[b + sz + sy + 1];
+        T.c[0] = g[b]; T.c[1] = g[b + 1];
+        T.c[2] = g[b + sy]; T.c[3] = g[b + sy + 1];
+        T.c[4] = g[b + sz]; T.c[5] = g[b + sz + 1];
+        T.c[6] = g[b + sz + sy]; T.c[7] = g[b + sz + sy + 1];
     }
+}
+
+__device__ __forceinline__ double phi_lerp(const PhiTap& T) {
+#pragma clang fp contract(off)
+    const double tx = T.tx, ty = T.ty, tz = T.tz;
+    const double c000 = T.c[0], c100 = T.c[1], c010 = T.c[2], c110 = T.c[3];
+    const double c001 = T.c[4], c101 = T.c[5], c011 = T.c[6], c111 = T.c[7];
     const double c00 = c000 * (1.0 - tx) + c100 * tx;
     const double c10 = c010 * (1.0 - tx) + c110 * tx;
     const double c01 = c001 * (1.0 - tx) + c101 * tx;
@@ -209,7 +224,13 @@ __device__ __forceinline__ double sample_phi(const DevComplex& C, double x, doub
     const double c0 = c00 * (1.0 - ty) + c10 * ty;
     const double c1 = c01 * (1.0 - ty) + c11 * ty;
     const double r = c0 * (1.0 - tz) + c1 * tz;
-    return in ? r : 0.0;
+    return T.in ? r : 0.0;
+}
+
+__device__ __forceinline__ double sample_phi(const DevComplex& C, double x, double y, double z) {
+    PhiTap T;
+    phi_fetch(C, x, y, z, T);
+    return phi_lerp(T);
 }
 
 // Monotone f32 key of an f64 squared distance that keeps membership of the

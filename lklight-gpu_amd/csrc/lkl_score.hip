@@ -548,9 +548,10 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
     if (tid == 0) next_group = 0u;
     constexpr int P1 = 3;
     for (int t0 = tid; t0 < nT; t0 += P1 * bs) {
-        double px[P1], py[P1], pz[P1], ph[P1];
+        double px[P1], py[P1], pz[P1];
         float q[P1];
         unsigned nb[P1], ne[P1];
+        PhiTap tap[P1];
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
             const int t = t0 + u * bs;
@@ -560,7 +561,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         }
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
-            ph[u] = FAR ? sample_phi(C, px[u], py[u], pz[u]) : 0.0;
+            if constexpr (FAR) phi_fetch(C, px[u], py[u], pz[u], tap[u]);
             const int n = list_node(C, px[u], py[u], pz[u]);
             const int m = n < 0 ? 0 : n;
             const unsigned b0 = C.noff[m], b1 = C.noff[m + 1];
@@ -571,7 +572,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         for (int u = 0; u < P1; ++u) {
             const int t = t0 + u * bs;
             if (t >= nT) break;
-            if (FAR && q[u] != 0.0f) e += to_fix((double)q[u] * ph[u], E_SCALE);
+            if (FAR && q[u] != 0.0f) e += to_fix((double)q[u] * phi_lerp(tap[u]), E_SCALE);
             ab[t] = nb[u];
             ae[t] = ne[u];
             // no receptor atom in reach: the desolvation minimum stays at its initial value
@@ -680,15 +681,19 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         };
         int id[GK];
         step_ids(b, id);
+        const char* __restrict__ rbase = reinterpret_cast<const char*>(TYPED ? C.rxyzq : C.rrec);
         for (unsigned p = b; __any(p < en); p += (unsigned)(GK * GL)) {
             float4 rq[GK], rp[GK];
 #pragma unroll
             for (int k = 0; k < GK; ++k) {
                 // with TYPED only the coordinates are gathered, from the dense
-                // (x, y, z, q) array, and the two LDS reads complete meanwhile
-                rq[k] = TYPED ? C.rxyzq[id[k]] : C.rrec[2 * id[k]];
+                // (x, y, z, q) array, and the two LDS reads complete meanwhile.
+                // Addresses are a uniform base plus a 32-bit byte offset (one
+                // shift per record; both halves of a record share it).
+                const unsigned off = (unsigned)id[k] << (TYPED ? 4 : 5);
+                rq[k] = *reinterpret_cast<const float4*>(rbase + off);
                 if constexpr (TYPED) rp[k] = ltab[reinterpret_cast<const unsigned char*>(ltyp)[id[k]]];
-                else rp[k] = C.rrec[2 * id[k] + 1];
+                else rp[k] = *reinterpret_cast<const float4*>(rbase + off + 16u);
             }
             int idn[GK];
             step_ids(p + (unsigned)(GK * GL), idn);
@@ -1771,6 +1776,8 @@ static lkl_ctx* ctx_create(const lkl_complex_desc* d) {
     if (!d) { set_err("lkl_ctx_create: null descriptor"); return nullptr; }
     if (d->nr < 0 || d->nl <= 0) { set_err("lkl_ctx_create: need nl > 0 and nr >= 0 (nr=%d nl=%d)", d->nr, d->nl); return nullptr; }
     if (d->nr > 0 && !d->r_xyz) { set_err("lkl_ctx_create: r_xyz is null"); return nullptr; }
+    // the list walk addresses receptor records by 32-bit byte offsets (32 B each)
+    if (d->nr >= (1 << 27)) { set_err("lkl_ctx_create: nr=%d exceeds the 134,217,727-atom receptor limit", d->nr); return nullptr; }
     if (!d->l_xyz) { set_err("lkl_ctx_create: l_xyz is null"); return nullptr; }
     if (!probe_device()) { set_err("lkl_ctx_create: no gfx950 device available"); return nullptr; }
     lkl_ctx* c = new lkl_ctx();
