@@ -369,6 +369,15 @@ def bench_c4(K, W, args, cpu):
         torch.cuda.synchronize()
         res[name] = time.perf_counter() - t0
     kern_ms = sum(a.elapsed_time(b) for a, b in ev) / K
+    # load balance of one traced resident launch (per-tile start/end and CU)
+    m.set_trace(True)
+    run_batch(W)
+    rec, khz = m.trace()
+    balance = lklight.load_balance(rec, khz)
+    m.set_trace(False)
+    log("c4: traced launch %d tiles on %d CUs, span %.1f us, CU busy %.3f, CU finish min/median/max %.0f/%.0f/%.0f us"
+        % (balance["tiles"], balance["cus"], balance["span_us"], balance["busy_frac"],
+           balance["cu_finish_us"]["min"], balance["cu_finish_us"]["median"], balance["cu_finish_us"]["max"]))
     same_seq = bool(torch.equal(d_out[W:], d_seq[W:]))
     same_host = bool(np.array_equal(d_out[W:].cpu().numpy(), np.stack(outs)))
     log("c4: host-buffer graph steps %.0f pose-evals/s, resident batch %.0f, sequential %.0f, kernel %.3f ms"
@@ -418,6 +427,7 @@ def bench_c4(K, W, args, cpu):
         "sequential_pose_evals_per_s": P * K / res["sequential"],
         "batch_over_sequential": res["sequential"] / res["batch"],
         "bitwise_equal_to_sequential": same_seq, "host_steps_equal_resident": same_host,
+        "load_balance": balance,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "traffic_source": TRAFFIC_FILE if traffic else None,
@@ -575,6 +585,8 @@ def main():
         "steady_state": head.get("steady_state"),
         "configs": {k: v for k, v in results.items() if k != head_key},
     }
+    if head.get("load_balance"):
+        out["load_balance"] = head["load_balance"]
     if "c2" in cache and not args.no_gso:
         cx = cache["c2"]
         ctx = lklight.Context(cx)

@@ -322,6 +322,17 @@ int lkl_multi_use_graph(lkl_multi* m, int enable);
  * to 64 rows (block, chunk_mult, median ms). */
 int lkl_multi_autotune(lkl_multi* m, const double* poses, int* best_block, int* best_mult,
                        float* table, int* n_table);
+/* Load-balance trace (no counterpart in the reference; instrumentation for
+ * the batch launch): with on, every following launch records per tile
+ * {start, end} on the device's constant wall clock, the CU it ran on
+ * (XCC, shader engine, CU as HIP's __smid) and its problem (batch order).
+ * lkl_multi_trace waits for the device (the launch may be on any stream) and
+ * copies the last launch's
+ * records, 4 x uint64 per tile in launch order (at most cap tiles), stores
+ * the clock's rate in kHz in *clock_khz (if not NULL) and returns the
+ * launch's tile count. */
+int lkl_multi_set_trace(lkl_multi* m, int on);
+int lkl_multi_trace(lkl_multi* m, unsigned long long* out, int cap, int* clock_khz);
 
 /* ── receptor far field and synthetic inputs ──────────────────────────────── */
 

@@ -965,11 +965,14 @@ struct ProblemDesc {
 // those of its own launch.  (No loop around barriers: a claim-loop variant
 // with a divergent tid==0 claim was restructured by the compiler into nested
 // loops that cycle the barriers without the claiming lane, which never ends.)
+// With trace set, each tile records {start, end} on the device's constant
+// 100 MHz wall clock, the CU it ran on (__smid: XCC, shader engine, CU) and
+// its problem, four 64-bit words per tile, for load-balance measurement.
 template <bool LIST>
 __global__ void __launch_bounds__(1024)
 multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__ probs, int K,
              int total_tiles, const double* __restrict__ poses, double* __restrict__ out,
-             long long* __restrict__ acc, unsigned* __restrict__ cnt) {
+             long long* __restrict__ acc, unsigned* __restrict__ cnt, unsigned long long* __restrict__ trace) {
     __shared__ long long red[2][16];
     extern __shared__ __attribute__((aligned(16))) unsigned char multi_lds[];  // work-ordered lists
     const int t = (int)blockIdx.x;
@@ -983,6 +986,11 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
     }
     const ProblemDesc P = probs[lo];
     const DevComplex C = cxs[lo];
+    if (trace && tid == 0) {
+        trace[4 * (size_t)t] = (unsigned long long)wall_clock64();
+        trace[4 * (size_t)t + 2] = __smid();
+        trace[4 * (size_t)t + 3] = (unsigned long long)lo;
+    }
     const int local = t - P.tile_begin;
     const int pose = P.pose_off + local / P.nchunks;
     const int j0 = (local % P.nchunks) * P.chunk;
@@ -1005,6 +1013,7 @@ multi_kernel(const DevComplex* __restrict__ cxs, const ProblemDesc* __restrict__
     if ((tid & 63) == 0) { red[0][w] = e; red[1][w] = v; }
     __syncthreads();
     if (tid != 0) return;
+    if (trace) trace[4 * (size_t)t + 1] = (unsigned long long)wall_clock64();
     long long te = 0, tv = 0;
     for (int k = 0; k < nw; ++k) { te += red[0][k]; tv += red[1][k]; }
     if (P.nchunks > 1) {
@@ -2597,6 +2606,9 @@ struct lkl_multi {
     double *d_poses = nullptr, *d_out = nullptr;
     long long* d_acc = nullptr;
     unsigned* d_cnt = nullptr;
+    unsigned long long* d_trace = nullptr;  // per-tile records of the last launch (lkl_multi_set_trace)
+    size_t cap_trace = 0;
+    bool trace_on = false;
     double *h_poses = nullptr, *h_out = nullptr;  // pinned staging
     bool use_graph = true, graph_ok = false;
     hipGraph_t graph = nullptr;
@@ -2615,7 +2627,7 @@ static void batch_free(lkl_multi* b) {
     if (!b) return;
     (void)hipSetDevice(b->device);
     batch_drop_graph(b);
-    void* bufs[] = {b->d_cx, b->d_prob, b->d_poses, b->d_out, b->d_acc, b->d_cnt};
+    void* bufs[] = {b->d_cx, b->d_prob, b->d_poses, b->d_out, b->d_acc, b->d_cnt, b->d_trace};
     for (void* p : bufs)
         if (p) (void)hipFree(p);
     if (b->h_poses) (void)hipHostFree(b->h_poses);
@@ -2661,6 +2673,7 @@ static int batch_refresh(lkl_multi* b) {
         b->versions[k] = c->version;
     }
     b->total_tiles = tile;
+    if (b->trace_on && dev_reserve(&b->d_trace, &b->cap_trace, 4 * (size_t)std::max(tile, 1))) return -1;
     HIPCK("upload", h2d(b->d_cx, cx.data(), sizeof(DevComplex) * b->K, b->stream));
     HIPCK("upload", h2d(b->d_prob, pd.data(), sizeof(ProblemDesc) * b->K, b->stream));
     batch_drop_graph(b);
@@ -2674,13 +2687,13 @@ static int batch_check(lkl_multi* b) {
 }
 
 typedef void (*multi_fn)(const DevComplex*, const ProblemDesc*, int, int, const double*, double*, long long*,
-                         unsigned*);
+                         unsigned*, unsigned long long*);
 
 static int batch_launch(lkl_multi* b, const double* d_poses, double* d_out, hipStream_t st) {
     if (b->total_tiles <= 0) return 0;
     multi_fn fn = b->list ? multi_kernel<true> : multi_kernel<false>;
     hipLaunchKernelGGL(fn, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, st, b->d_cx,
-                       b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt);
+                       b->d_prob, b->K, b->total_tiles, d_poses, d_out, b->d_acc, b->d_cnt, b->d_trace);
     HIPCK("launch", hipGetLastError());
     return 0;
 }
@@ -2760,6 +2773,39 @@ LKL_API int lkl_multi_use_graph(lkl_multi* b, int enable) {
     return 0;
 }
 
+// Per-tile trace of every following launch (on) or none (off, the default).
+// Switching drops the captured graph (its launch carries the trace pointer).
+LKL_API int lkl_multi_set_trace(lkl_multi* b, int on) {
+    if (!b) { set_err("lkl_multi_set_trace: null batch"); return -1; }
+    HIPCK("probe", hipSetDevice(b->device));
+    if ((on != 0) == b->trace_on) return 0;
+    b->trace_on = on != 0;
+    if (b->trace_on) {
+        if (dev_reserve(&b->d_trace, &b->cap_trace, 4 * (size_t)std::max(b->total_tiles, 1))) return -1;
+    } else {
+        (void)hipFree(b->d_trace);
+        b->d_trace = nullptr;
+        b->cap_trace = 0;
+    }
+    batch_drop_graph(b);
+    return 0;
+}
+
+// The last traced launch's records (blocking: waits for the whole device, as
+// the launch may have been enqueued on any stream): 4 words per tile, tiles
+// in launch order; returns the tile count (at most cap records copied) or -1.
+// clock_khz (may be NULL) receives the wall clock's rate.
+LKL_API int lkl_multi_trace(lkl_multi* b, unsigned long long* out, int cap, int* clock_khz) {
+    if (!b || !out || cap < 0) { set_err("lkl_multi_trace: bad arguments"); return -1; }
+    if (!b->trace_on) { set_err("lkl_multi_trace: tracing is off (lkl_multi_set_trace)"); return -1; }
+    HIPCK("probe", hipSetDevice(b->device));
+    if (clock_khz) HIPCK("probe", hipDeviceGetAttribute(clock_khz, hipDeviceAttributeWallClockRate, b->device));
+    HIPCK("sync", hipDeviceSynchronize());
+    const int n = std::min(cap, b->total_tiles);
+    HIPCK("readback", hipMemcpy(out, b->d_trace, sizeof(unsigned long long) * 4 * (size_t)n, hipMemcpyDeviceToHost));
+    return b->total_tiles;
+}
+
 // Device buffers: poses (P x 7, batch order), out (P x 2).  One kernel launch.
 LKL_API int lkl_multi_score_async(lkl_multi* b, const double* d_poses, double* d_out, void* stream) {
     if (!b || !d_poses || !d_out) { set_err("lkl_multi_score_async: bad arguments"); return -1; }
@@ -2784,7 +2830,7 @@ LKL_API int lkl_multi_score(lkl_multi* b, const double* poses, double* out) {
             multi_fn fn = b->list ? multi_kernel<true> : multi_kernel<false>;
             hipLaunchKernelGGL(fn, dim3((unsigned)b->total_tiles), dim3((unsigned)b->block), b->lds, b->stream,
                                b->d_cx, b->d_prob, b->K, b->total_tiles, b->d_poses, b->d_out, b->d_acc,
-                               b->d_cnt);
+                               b->d_cnt, b->d_trace);
         }
         hipError_t e2 = hipGetLastError();
         hipError_t e3 = hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 2 * P,

@@ -316,6 +316,23 @@ class Multi:
     def use_graph(self, on):
         lib().lkl_multi_use_graph(self.h, int(bool(on)))
 
+    def set_trace(self, on):
+        """lkl_multi_set_trace: record per-tile start/end, CU and problem of each launch."""
+        _capi.check(lib().lkl_multi_set_trace(self.h, int(bool(on))), "lkl_multi_set_trace")
+
+    def trace(self):
+        """The last traced launch: (records uint64 [tiles, 4] = start, end, CU id, problem;
+        wall-clock rate in kHz)."""
+        khz = C.c_int()
+        rec = np.zeros((1, 4), np.uint64)
+        n = lib().lkl_multi_trace(self.h, _capi.ptr(rec, C.c_uint64), 0, C.byref(khz))  # the tile count
+        if n >= 0:
+            rec = np.zeros((max(n, 1), 4), np.uint64)
+            n = lib().lkl_multi_trace(self.h, _capi.ptr(rec, C.c_uint64), n, C.byref(khz))
+        if n < 0:
+            raise _capi.GpuError("lkl_multi_trace failed: " + _capi.last_error())
+        return rec[:n], khz.value
+
     def set_tiles(self, block, chunk_mult):
         _capi.check(lib().lkl_multi_set_tiles(self.h, int(block), int(chunk_mult)),
                     "lkl_multi_set_tiles")
@@ -328,3 +345,40 @@ class Multi:
                                              _capi.ptr(table, C.c_float), C.byref(n)),
                     "lkl_multi_autotune")
         return b.value, m.value, table[:n.value].tolist()
+
+
+def load_balance(rec, clock_khz):
+    """Load balance of one traced batch launch (Multi.trace()): per CU the union
+    of its tiles' [start, end) intervals is its busy time.  Returns the span
+    (first start to last end), the busy fraction (sum of CU busy times over
+    CUs x span), the mean number of tiles in flight per CU, and the spread of
+    the CUs' last finish times, all times in us."""
+    r = np.asarray(rec, dtype=np.int64)
+    if r.shape[0] == 0:
+        return None
+    us = 1000.0 / float(clock_khz)
+    start, end, cu = r[:, 0], r[:, 1], r[:, 2]
+    t0, t1 = int(start.min()), int(end.max())
+    span = float(t1 - t0)
+    busy, fin = [], []
+    for c in np.unique(cu):
+        m = cu == c
+        iv = sorted(zip(start[m].tolist(), end[m].tolist()))
+        tot, cs, ce = 0, iv[0][0], iv[0][1]
+        for a, b in iv[1:]:
+            if a > ce:
+                tot += ce - cs
+                cs, ce = a, b
+            else:
+                ce = max(ce, b)
+        tot += ce - cs
+        busy.append(tot)
+        fin.append(int(end[m].max()) - t0)
+    busy, fin = np.array(busy, float), np.array(fin, float)
+    ncu = len(busy)
+    return {"tiles": int(r.shape[0]), "cus": ncu, "span_us": span * us,
+            "busy_frac": float(busy.sum() / (ncu * span)),
+            "tiles_in_flight_per_cu": float((end - start).sum() / (ncu * span)),
+            "cu_finish_us": {"min": float(fin.min()) * us, "median": float(np.median(fin)) * us,
+                             "max": float(fin.max()) * us},
+            "tile_us": {"median": float(np.median(end - start)) * us, "max": float((end - start).max()) * us}}

@@ -128,6 +128,8 @@ def lib():
         "lkl_multi_score": (C.c_int, [C.c_void_p, _d, _d]),
         "lkl_multi_score_async": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
         "lkl_multi_use_graph": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_multi_set_trace": (C.c_int, [C.c_void_p, C.c_int]),
+        "lkl_multi_trace": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.c_int, _i]),
         "lkl_multi_autotune": (C.c_int, [C.c_void_p, _d, _i, _i, _f, _i]),
         "lkl_gso_create": (C.c_void_p, [C.c_void_p, C.c_int, C.c_int, _i, _d, C.c_uint64,
                                         SCORE_FALLBACK, C.c_void_p]),

@@ -104,3 +104,15 @@ def test_synth_generator():
     assert np.allclose(a.lig.xyz.mean(0), 0, atol=1e-3)
     p = synth.make_poses(50, ax, 3)
     assert np.allclose(np.linalg.norm(p[:, :4], axis=1), 1, atol=1e-6)
+
+
+def test_load_balance_intervals():
+    """Busy time per CU is the union of its tiles' intervals; times scale by
+    the clock rate (kHz)."""
+    rec = np.array([[10, 110, 7, 0], [60, 160, 7, 0], [210, 310, 7, 1], [10, 310, 9, 1]], np.uint64)
+    lb = lklight.load_balance(rec, 100_000)  # 100 MHz: 100 ticks = 1 us
+    assert lb["cus"] == 2 and lb["tiles"] == 4
+    assert math.isclose(lb["span_us"], 3.0)
+    assert math.isclose(lb["busy_frac"], (150 + 100 + 300) / (2 * 300))
+    assert math.isclose(lb["tiles_in_flight_per_cu"], 600 / 600)
+    assert math.isclose(lb["cu_finish_us"]["min"], 3.0) and math.isclose(lb["tile_us"]["max"], 3.0)

@@ -4,6 +4,7 @@ also when replayed from a hipGraph."""
 import sys
 
 import numpy as np
+import torch
 import pytest
 
 import lklight
@@ -62,6 +63,40 @@ def test_multi_matches_single(batch):
         assert np.array_equal(m.score(allp), ref), (block, mult)
     # a re-tiled context is picked up by the batch
     ctxs[0].set_tiles(6.0, 128, 0)
+    assert np.array_equal(m.score(allp), ref)
+    m.close()
+
+
+def test_multi_trace(batch):
+    """A traced launch scores the same bits and records every tile once: start
+    before end on the wall clock, a CU id, and per problem as many tiles as it
+    has poses (whole-ligand tiles)."""
+    cxs, ctxs, flags, poses = batch
+    ref = np.concatenate([np.stack(ctx.score(p, f)[:2], 1) for ctx, p, f in zip(ctxs, poses, flags)])
+    allp = np.concatenate(poses)
+    m = lklight.Multi(ctxs, flags, [p.shape[0] for p in poses])
+    m.set_tiles(256, 0)
+    m.set_trace(True)
+    for rep in range(2):  # the graph is captured with the trace buffer, then replayed
+        assert np.array_equal(m.score(allp), ref), rep
+        rec, khz = m.trace()
+        assert khz > 0 and rec.shape == (allp.shape[0], 4)
+        assert (rec[:, 0] <= rec[:, 1]).all() and (rec[:, 0] > 0).all()
+        counts = np.bincount(rec[:, 3].astype(np.int64), minlength=len(poses))
+        assert sorted(counts.tolist()) == sorted(p.shape[0] for p in poses)
+        lb = lklight.load_balance(rec, khz)
+        say("multi trace: %d tiles on %d CUs, span %.1f us, busy %.3f"
+            % (lb["tiles"], lb["cus"], lb["span_us"], lb["busy_frac"]))
+        assert 0.0 < lb["busy_frac"] <= 1.0 and lb["cus"] >= 1
+    # a launch on another stream: the readback waits for it
+    d_p = torch.from_numpy(allp).cuda()
+    d_o = torch.zeros((allp.shape[0], 2), dtype=torch.float64, device="cuda")
+    st = torch.cuda.Stream()
+    m.score_async(d_p.data_ptr(), d_o.data_ptr(), st.cuda_stream)
+    rec, khz = m.trace()
+    assert (rec[:, 0] > 0).all() and (rec[:, 0] <= rec[:, 1]).all()
+    assert np.array_equal(d_o.cpu().numpy(), ref)
+    m.set_trace(False)
     assert np.array_equal(m.score(allp), ref)
     m.close()
 
