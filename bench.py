@@ -678,6 +678,41 @@ def gso_step_compare(cx, ctx, family, steps=50):
     return res
 
 
+def library_mlp_reference(rows, reps=10):
+    """The vendor library's time for the screen's forward shapes (torch.matmul
+    in bf16, which calls hipBLASLt): rows x 6144 -> 512 -> 512 -> 512 with bias
+    and ReLU, random weights.  A reference point for the hand-written MFMA
+    GEMMs, not used by the screen."""
+    import torch
+    dev = torch.device("cuda:0")
+    x = torch.randn(rows, 6144, dtype=torch.bfloat16, device=dev)
+    w = [torch.randn(512, k, dtype=torch.bfloat16, device=dev) * 0.01 for k in (6144, 512, 512)]
+    bias = torch.zeros(512, dtype=torch.bfloat16, device=dev)
+
+    def layer1():
+        return torch.relu(x @ w[0].t() + bias)
+
+    def forward():
+        h = layer1()
+        for k in (1, 2):
+            h = torch.relu(h @ w[k].t() + bias)
+        return h
+    out = {}
+    for name, fn in (("forward_ms", forward), ("layer1_ms", layer1)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = e0.elapsed_time(e1) / reps
+    out["layer1_tflops"] = 2.0 * rows * 512 * 6144 / (out["layer1_ms"] * 1e-3) / 1e12
+    out["rows"] = rows
+    return out
+
+
 def surrogate_experiment(args):
     """Config-5 comparison on the 1,500-atom / 800-atom complex: train the
     surrogate on the GPU from exact dna scores, then, from the same starts
@@ -758,6 +793,9 @@ def surrogate_experiment(args):
     per_exact = cost["exact_ms"] / nc * 1e6
     log("per candidate: features + MLP %.1f ns, exact score %.1f ns" % (per_cand, per_exact))
     del Xc, yc, d_o, cand
+    lib_ref = library_mlp_reference((nc + 127) // 128 * 128)
+    log("library (torch/hipBLASLt) bf16 forward of the same shapes: %.3f ms, layer 1 %.0f TFLOP/s"
+        % (lib_ref["forward_ms"], lib_ref["layer1_tflops"]))
 
     def plain_device(pos, seed):
         g = G.GSO(ctx, method, pos, seed, sizes)
@@ -831,6 +869,7 @@ def surrogate_experiment(args):
         "reaches_plain_top10": [r["screened"]["reached_step"] is not None for r in runs],
         "spearman_heldout": rho, "spearman_within_glowworm": rho_pg,
         "cost_per_batch": dict(cost, candidates=nc),
+        "library_forward_reference": lib_ref,
         "break_even": {
             "surrogate_ns_per_candidate": per_cand, "exact_ns_per_pose": per_exact,
             "candidate_over_exact": per_cand / per_exact,
