@@ -15,15 +15,22 @@
 //   fill   the same stream writes each neighbour as (z, R, |dxy|, angle), in
 //          index order; angle = atan2(dy, dx) + pi seen from the atom, which the
 //          reference recomputes per slice with the same operands.
-//   area   one lane per atom, slices in order; buried arcs go to the atom's own
-//          scratch region, are insertion-sorted by start and summed in sorted
-//          order (lr_sasa.rs:176-274).  Sorting makes the result independent of
-//          neighbour order, so index order gives the reference's areas.
+//   area   one wavefront per atom, slices in order: on each slice the lanes
+//          form the buried arcs of the atom's neighbours in parallel (lane k:
+//          neighbours k, k + 64), a wave vote takes the slice out when one of
+//          them buries it, the arcs are placed in LDS in order of their start
+//          by a rank count (ties by slot), and one lane sweeps the sorted arcs
+//          (lr_sasa.rs:176-213, the reference's additions in its order).
+//          Atoms with more than 128 neighbours take one lane and a scratch
+//          region of their own, insertion-sorted (the same sweep).  The
+//          sweep's result does not depend on the order of arcs with equal
+//          starts, so both give the same bits, and the reference's areas.
 // The device acos/atan2 are within an ulp or two of the host libm's, so areas
 // agree with a CPU evaluation to ~1e-15 relative, not bit for bit.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <vector>
 
@@ -95,16 +102,41 @@ __device__ double exposed_arc_length(double* arc, int n) {
     return sum + TWOPI - sup;
 }
 
-__global__ void __launch_bounds__(TILE)
-sasa_area_kernel(const double4* __restrict__ at, int n, const long long* __restrict__ off,
-                 const double4* __restrict__ nbr, double* __restrict__ arcs, int ns,
-                 double* __restrict__ out) {
+// The arc (or two, when it wraps past 2 pi) that neighbour q buries on the
+// slice circle of radius ri_p at height z (lr_sasa.rs:225-262); returns the
+// number of arcs (0: q misses the circle or lies inside it), or -1 when q
+// buries the whole circle.
+__device__ __forceinline__ int slice_arcs(const double4 q, double z, double ri_p, double ri_p2, double* a) {
 #pragma clang fp contract(off)
-    const int i = blockIdx.x * TILE + threadIdx.x;
-    if (i >= n) return;
-    const double zi = at[i].z, ri = at[i].w;
-    const long long b = off[i];
-    const int nni = (int)(off[i + 1] - b);
+    const double dj = fabs(q.x - z), rj = q.y;
+    if (!(dj < rj)) return 0;
+    const double rj_p2 = rj * rj - dj * dj;
+    const double rj_p = sqrt(rj_p2);
+    const double dij = q.z;
+    if (dij >= ri_p + rj_p) return 0;
+    if (dij + ri_p < rj_p) return -1;
+    if (dij + rj_p < ri_p) return 0;
+    const double alpha = acos((ri_p2 + dij * dij - rj_p2) / (2.0 * ri_p * dij));
+    double inf = q.w - alpha, sup = q.w + alpha;
+    if (inf < 0.0) inf += TWOPI;
+    if (sup > TWOPI) sup -= TWOPI;
+    if (sup < inf) {
+        a[0] = 0.0;
+        a[1] = sup;
+        a[2] = inf;
+        a[3] = TWOPI;
+        return 2;
+    }
+    a[0] = inf;
+    a[1] = sup;
+    return 1;
+}
+
+// One atom on one lane: arcs in the atom's own scratch region.
+__device__ double area_one_lane(const double4 ai, long long b, int nni, const double4* __restrict__ nbr,
+                                double* __restrict__ arcs, int ns) {
+#pragma clang fp contract(off)
+    const double zi = ai.z, ri = ai.w;
     double* arc = arcs + 4 * b;
     const double delta = 2.0 * ri / (double)ns;
     double z = zi - ri - 0.5 * delta;
@@ -119,34 +151,105 @@ sasa_area_kernel(const double4* __restrict__ at, int n, const long long* __restr
         int na = 0;
         bool buried = false;
         for (int k = 0; k < nni; ++k) {
-            const double4 q = nbr[b + k];  // z, R, |dxy|, angle
-            const double dj = fabs(q.x - z), rj = q.y;
-            if (!(dj < rj)) continue;
-            const double rj_p2 = rj * rj - dj * dj;
-            const double rj_p = sqrt(rj_p2);
-            const double dij = q.z;
-            if (dij >= ri_p + rj_p) continue;
-            if (dij + ri_p < rj_p) { buried = true; break; }
-            if (dij + rj_p < ri_p) continue;
-            const double alpha = acos((ri_p2 + dij * dij - rj_p2) / (2.0 * ri_p * dij));
-            double inf = q.w - alpha, sup = q.w + alpha;
-            if (inf < 0.0) inf += TWOPI;
-            if (sup > TWOPI) sup -= TWOPI;
-            if (sup < inf) {
-                arc[2 * na] = 0.0;
-                arc[2 * na + 1] = sup;
-                arc[2 * na + 2] = inf;
-                arc[2 * na + 3] = TWOPI;
-                na += 2;
-            } else {
-                arc[2 * na] = inf;
-                arc[2 * na + 1] = sup;
-                na += 1;
-            }
+            const int c = slice_arcs(nbr[b + k], z, ri_p, ri_p2, arc + 2 * na);
+            if (c < 0) { buried = true; break; }
+            na += c;
         }
         if (!buried) sasa += delta * ri * exposed_arc_length(arc, na);
     }
-    out[i] = sasa;
+    return sasa;
+}
+
+// Wavefront per atom (see the file comment).  SW atoms per workgroup, each
+// wave on its own LDS; no workgroup barrier (the waves take their own paths):
+// lanes of one wave meet in LDS through a wave-local fence.
+constexpr int SW = 4;
+constexpr int ARC_CAP = 256;  // arc slots per slice: two per neighbour, 128 neighbours
+
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // this wave's LDS accesses complete
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ void __launch_bounds__(64 * SW)
+sasa_area_wave_kernel(const double4* __restrict__ at, int n, const long long* __restrict__ off,
+                      const double4* __restrict__ nbr, double* __restrict__ arcs, int ns, int lane_only,
+                      double* __restrict__ out) {
+#pragma clang fp contract(off)
+    __shared__ double2 slot[SW][ARC_CAP];    // arcs by neighbour slot (start = +inf: empty)
+    __shared__ double2 sorted[SW][ARC_CAP];  // the same arcs by start
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int i = (int)blockIdx.x * SW + w;
+    if (i >= n) return;  // uniform per wave
+    const double4 ai = at[i];
+    const long long b = off[i];
+    const int nni = (int)(off[i + 1] - b);
+    if (lane_only || 2 * nni > ARC_CAP) {
+        if (lane == 0) out[i] = area_one_lane(ai, b, nni, nbr, arcs, ns);
+        return;
+    }
+    const double zi = ai.z, ri = ai.w;
+    const double delta = 2.0 * ri / (double)ns;
+    double z = zi - ri - 0.5 * delta;
+    double sasa = 0.0;
+    const int nslot = 2 * nni;
+    for (int s = 0; s < ns; ++s) {
+        z += delta;
+        const double di = fabs(zi - z);
+        const double ri_p2 = ri * ri - di * di;
+        if (ri_p2 < 0.0) continue;
+        const double ri_p = sqrt(ri_p2);
+        if (ri_p <= 0.0) continue;
+        // arcs of neighbours lane and lane + 64, two slots each
+        bool buried = false;
+        for (int k = lane; k < nni; k += 64) {
+            double a[4];
+            const int c = slice_arcs(nbr[b + k], z, ri_p, ri_p2, a);
+            buried |= c < 0;
+            slot[w][2 * k] = c >= 1 ? make_double2(a[0], a[1]) : make_double2(INFINITY, INFINITY);
+            slot[w][2 * k + 1] = c == 2 ? make_double2(a[2], a[3]) : make_double2(INFINITY, INFINITY);
+        }
+        if (__any(buried)) {
+            wave_sync_lds();  // the slots are rewritten on the next slice
+            continue;
+        }
+        wave_sync_lds();
+        // rank by (start, slot): the arcs' order of start, ties by slot.  Starts
+        // are +0 .. 2 pi or +inf (empty), so their bit patterns order like the
+        // values; a NaN bound (a degenerate tangency) sorts after +inf and is
+        // dropped with the empty slots.
+        int na = 0;
+        for (int e = lane; e < nslot; e += 64) {
+            const double2 me = slot[w][e];
+            const unsigned long long ke = (unsigned long long)__double_as_longlong(me.x);
+            int rank = 0;
+            for (int f = 0; f < nslot; ++f) {
+                const unsigned long long kf = (unsigned long long)__double_as_longlong(slot[w][f].x);
+                rank += (kf < ke || (kf == ke && f < e)) ? 1 : 0;
+            }
+            sorted[w][rank] = me;
+            na += me.x < INFINITY ? 1 : 0;
+        }
+        for (int o = 32; o > 0; o >>= 1) na += __shfl_xor(na, o, 64);
+        wave_sync_lds();
+        double exposed = 0.0;
+        if (lane == 0) {  // lr_sasa.rs:176-213 over the sorted arcs
+            if (na == 0) {
+                exposed = TWOPI;
+            } else {
+                double sum = sorted[w][0].x, sup = sorted[w][0].y;
+                for (int k = 1; k < na; ++k) {
+                    const double2 q = sorted[w][k];
+                    if (sup < q.x) sum += q.x - sup;
+                    if (q.y > sup) sup = q.y;
+                }
+                exposed = sum + TWOPI - sup;
+            }
+            sasa += delta * ri * exposed;
+        }
+        wave_sync_lds();  // sorted[] is rewritten on the next slice
+    }
+    if (lane == 0) out[i] = sasa;
 }
 
 }  // namespace
@@ -222,7 +325,11 @@ LKL_API int lkl_lr_sasa(const double* xyz, const double* radii, int n, double pr
         return fail("upload", e);
     hipLaunchKernelGGL(sasa_pairs_kernel<true>, grid, dim3(TILE), 0, st, d_at, n, d_off, nullptr, d_nbr);
     if ((e = hipGetLastError()) != hipSuccess) return fail("fill", e);
-    hipLaunchKernelGGL(sasa_area_kernel, grid, dim3(TILE), 0, st, d_at, n, d_off, d_nbr, d_arc, n_slices, d_out);
+    // LKL_SASA_LANE=1 runs every atom on one lane (the path of atoms with more
+    // than 128 neighbours), for tests that compare the two paths
+    const int lane_only = getenv("LKL_SASA_LANE") && getenv("LKL_SASA_LANE")[0] == '1' ? 1 : 0;
+    hipLaunchKernelGGL(sasa_area_wave_kernel, dim3((unsigned)((n + SW - 1) / SW)), dim3(64 * SW), 0, st, d_at, n,
+                       d_off, d_nbr, d_arc, n_slices, lane_only, d_out);
     if ((e = hipGetLastError()) != hipSuccess) return fail("area", e);
     if ((e = hipMemcpyAsync(out, d_out, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, st)) != hipSuccess)
         return fail("readback", e);

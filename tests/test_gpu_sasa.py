@@ -61,6 +61,28 @@ def test_receptor_of_8000_atoms(gpu):
     assert (g > 0).mean() > 0.3
 
 
+def test_wave_path_equals_lane_path(gpu):
+    """The wavefront-per-atom areas (arcs formed in parallel, ranked in LDS)
+    equal, bit for bit, those of the one-lane-per-atom path (arcs insertion-
+    sorted in scratch), which atoms with more than 128 neighbours take."""
+    import os
+    xyz, _ = synth.rsa(8000, synth.scaled_axes(8000, (1.0, 1.0, 1.0)), 42)
+    radii = np.random.default_rng(4).uniform(1.5, 2.0, 8000)
+    z = np.load(A.PATH)
+    heavy = z["rec_cpy_hyd"] == 1
+    cases = [(xyz, radii, 1.4), (z["rec_xyz"][heavy], z["rec_cpy_desrad"][heavy], 1.4),
+             (xyz[:3000], radii[:3000] + 1.0, 3.0)]  # large spheres: many neighbours
+    for x, r, probe in cases:
+        wave = lklight.lr_sasa(x, r, probe)
+        os.environ["LKL_SASA_LANE"] = "1"
+        try:
+            lane = lklight.lr_sasa(x, r, probe)
+        finally:
+            del os.environ["LKL_SASA_LANE"]
+        say("wave vs lane path: n %d probe %.1f, bitwise equal %.4f" % (len(r), probe, float(np.mean(wave == lane))))
+        assert np.array_equal(wave, lane)
+
+
 def test_bad_arguments_are_refused(gpu):
     assert lklight.lr_sasa(np.zeros((0, 3)), np.zeros(0)).shape == (0,)
     with pytest.raises(lklight.GpuError):
