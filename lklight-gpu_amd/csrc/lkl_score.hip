@@ -546,6 +546,12 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         for (int k = tid; k < nwd; k += bs) ltyp[k] = C.rtyp[k];
     }
     if (tid == 0) next_group = 0u;
+    // Without desolvation the far field is sampled after the compaction by
+    // the tile's last wavefront alone (PHI_W), which then joins the walk: the
+    // random phi reads wait on that wave while the others walk.  The
+    // desolvation variants, at their register cap, sample it here in pass 1.
+    constexpr bool PHI_W = FAR && DES == DES_NONE;
+    constexpr int PHI_U = 2;  // atoms per lane in flight (more spill the walk's registers, and gain nothing)
     constexpr int P1 = 3;
     for (int t0 = tid; t0 < nT; t0 += P1 * bs) {
         double px[P1], py[P1], pz[P1];
@@ -561,7 +567,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         }
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
-            if constexpr (FAR) phi_fetch(C, px[u], py[u], pz[u], tap[u]);
+            if constexpr (FAR && !PHI_W) phi_fetch(C, px[u], py[u], pz[u], tap[u]);
             const int n = list_node(C, px[u], py[u], pz[u]);
             const int m = n < 0 ? 0 : n;
             const unsigned b0 = C.noff[m], b1 = C.noff[m + 1];
@@ -572,7 +578,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         for (int u = 0; u < P1; ++u) {
             const int t = t0 + u * bs;
             if (t >= nT) break;
-            if (FAR && q[u] != 0.0f) e += to_fix((double)q[u] * phi_lerp(tap[u]), E_SCALE);
+            if (FAR && !PHI_W && q[u] != 0.0f) e += to_fix((double)q[u] * phi_lerp(tap[u]), E_SCALE);
             ab[t] = nb[u];
             ae[t] = ne[u];
             // no receptor atom in reach: the desolvation minimum stays at its initial value
@@ -632,6 +638,26 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
             __syncthreads();
         }
         for (int q = 0; q < GB; ++q) nact += brun[q];
+    }
+    if constexpr (PHI_W) {
+        if (w == nw - 1) {  // PHI_U atoms per lane in flight, transformed again from their records
+            for (int t0 = lane; t0 < nT; t0 += PHI_U * 64) {
+                PhiTap tp[PHI_U];
+                float qq[PHI_U];
+#pragma unroll
+                for (int u = 0; u < PHI_U; ++u) {
+                    const int t = t0 + 64 * u;
+                    const float4 lb = C.lxyzq[j0 + (t < nT ? t : t0)];
+                    double x, y, z;
+                    transform(M, lb.x, lb.y, lb.z, x, y, z);
+                    qq[u] = t < nT ? lb.w : 0.0f;
+                    phi_fetch(C, x, y, z, tp[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < PHI_U; ++u)
+                    if (qq[u] != 0.0f) e += to_fix((double)qq[u] * phi_lerp(tp[u]), E_SCALE);
+            }
+        }
     }
     const unsigned ngroups = (nact + (unsigned)GA - 1u) / (unsigned)GA;
     const int sub = lane / GL, r = lane % GL;
