@@ -546,18 +546,15 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         for (int k = tid; k < nwd; k += bs) ltyp[k] = C.rtyp[k];
     }
     if (tid == 0) next_group = 0u;
-    // Without desolvation the far field is sampled after the compaction by
-    // the tile's last wavefront alone (PHI_W), which then joins the walk: the
-    // random phi reads wait on that wave while the others walk.  The
-    // desolvation variants, at their register cap, sample it here in pass 1.
-    constexpr bool PHI_W = FAR && DES == DES_NONE;
+    // The far field is sampled after the compaction by the tile's last quarter
+    // of wavefronts (at least one), which then join the walk: the random phi
+    // reads wait on those waves while the others walk.
     constexpr int PHI_U = 2;  // atoms per lane in flight (more spill the walk's registers, and gain nothing)
     constexpr int P1 = 3;
     for (int t0 = tid; t0 < nT; t0 += P1 * bs) {
         double px[P1], py[P1], pz[P1];
         float q[P1];
         unsigned nb[P1], ne[P1];
-        PhiTap tap[P1];
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
             const int t = t0 + u * bs;
@@ -567,7 +564,6 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         }
 #pragma unroll
         for (int u = 0; u < P1; ++u) {
-            if constexpr (FAR && !PHI_W) phi_fetch(C, px[u], py[u], pz[u], tap[u]);
             const int n = list_node(C, px[u], py[u], pz[u]);
             const int m = n < 0 ? 0 : n;
             const unsigned b0 = C.noff[m], b1 = C.noff[m + 1];
@@ -578,7 +574,6 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         for (int u = 0; u < P1; ++u) {
             const int t = t0 + u * bs;
             if (t >= nT) break;
-            if (FAR && !PHI_W && q[u] != 0.0f) e += to_fix((double)q[u] * phi_lerp(tap[u]), E_SCALE);
             ab[t] = nb[u];
             ae[t] = ne[u];
             // no receptor atom in reach: the desolvation minimum stays at its initial value
@@ -639,9 +634,10 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         }
         for (int q = 0; q < GB; ++q) nact += brun[q];
     }
-    if constexpr (PHI_W) {
-        if (w == nw - 1) {  // PHI_U atoms per lane in flight, transformed again from their records
-            for (int t0 = lane; t0 < nT; t0 += PHI_U * 64) {
+    if constexpr (FAR) {
+        const int pw = max(1, nw / 4);
+        if (w >= nw - pw) {  // PHI_U atoms per lane in flight, transformed again from their records
+            for (int t0 = lane + (w - (nw - pw)) * 64 * PHI_U; t0 < nT; t0 += PHI_U * 64 * pw) {
                 PhiTap tp[PHI_U];
                 float qq[PHI_U];
 #pragma unroll
