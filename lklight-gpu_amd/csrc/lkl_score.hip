@@ -1146,10 +1146,12 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
 // lane places U atoms, then requests their offsets together.
 constexpr int COST_STRIDE = 8;
 __global__ void __launch_bounds__(64) pose_cost_kernel(DevComplex C, const double* __restrict__ poses, int N,
-                                                       unsigned* __restrict__ cost) {
+                                                       unsigned* __restrict__ cost, double* __restrict__ pose_copy) {
     const int pose = blockIdx.x;
     if (pose >= N) return;
     const Mat M = pose_matrix(poses + (size_t)pose * 7);
+    // poses read from host memory are kept in device memory for the scoring kernel
+    if (pose_copy && threadIdx.x < 7) pose_copy[(size_t)pose * 7 + threadIdx.x] = poses[(size_t)pose * 7 + threadIdx.x];
     const float4* lx = C.lxyzq + (C.lstride ? (size_t)pose * (size_t)C.lstride : 0);
     constexpr int U = 4;
     unsigned c = 0;
@@ -1317,7 +1319,8 @@ struct lkl_ctx {
     size_t cap_acc = 0;
     // pinned host staging of the blocking call (poses in, scores out): one
     // DMA each way and a single stream synchronisation per call
-    double* h_stage = nullptr;
+    double* h_stage = nullptr;  // pinned, device-mapped staging of the host-buffer call
+    double* d_stage = nullptr;  // its device address
     size_t cap_stage = 0;
     // optional kernel timing of every launch: events before the dispatch-order
     // pre-pass, before the scoring kernel and after it, on the launch's stream
@@ -1956,7 +1959,7 @@ static int reserve_split_min(lkl_ctx* c, size_t n) {
 // Enqueue one score launch; all pointers are device pointers.
 static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags, double* d_out,
                         double* d_outS, hipStream_t st, bool peratom, const float4* d_lcoords = nullptr,
-                        const LaunchScratch* ext = nullptr) {
+                        const LaunchScratch* ext = nullptr, double* pose_copy = nullptr) {
     if (N <= 0) return 0;
     if (validate_flags(c, flags)) return -1;
     DevComplex C = dev_view(c);
@@ -2039,13 +2042,14 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         if (!ext && dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
         unsigned* pcost = ext ? ext->pcost : c->d_pcost;
         int* porder = ext ? ext->porder : c->d_porder;
-        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, pcost);
+        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, pcost, pose_copy);
         int pm = 64;  // threads of the sort: the range rounded up to a power of two, at most 1024
         while (pm < (N + 7) / 8 && pm < 1024) pm <<= 1;
         hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, pcost, N, porder);
         HIPCK("launch", hipGetLastError());
         C.porder = porder;
         c->ev_prepass = true;
+        if (pose_copy) d_poses = pose_copy;
     }
     if (c->timing) (void)hipEventRecord(c->ev[1], st);
     hipLaunchKernelGGL(fn, dim3((unsigned)(N * nchunks)), dim3((unsigned)c->block), shm, st, C,
@@ -2055,7 +2059,13 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     return 0;
 }
 
-// Blocking host-buffer score: upload poses, one launch, read back.
+// Blocking host-buffer score.  The poses go into a pinned, device-mapped
+// staging buffer that the kernels read over the bus: the dispatch-order
+// pre-pass keeps a device copy of each pose it reads for the scoring kernel
+// (without the pre-pass the scoring kernel reads the staging buffer itself),
+// and the scoring kernel stores the scores straight into the staging buffer.
+// No DMA copies: one launch sequence and one stream synchronisation per call.
+// Staging layout: N x 7 poses, then N x 2 scores and N desolvation terms.
 static int score_host(lkl_ctx* c, const double* poses, int N, unsigned flags, double* out,
                       double* outS, float* pa_e, float* pa_v) {
     if (N <= 0) { set_err("N must be positive (N=%d)", N); return -1; }
@@ -2064,43 +2074,37 @@ static int score_host(lkl_ctx* c, const double* poses, int N, unsigned flags, do
     HIPCK("probe", hipSetDevice(c->device));
     (void)hipGetLastError();  // drain a stale sticky error (full_score.cu:375)
     if (dev_reserve(&c->d_poses, &c->cap_poses, (size_t)N * 7)) return -1;
-    if (dev_reserve(&c->d_out, &c->cap_out, (size_t)N * 2)) return -1;
-    if ((flags & F_DESOLV) && dev_reserve(&c->d_outS, &c->cap_outS, (size_t)N)) return -1;
-    // pinned staging: N x 7 poses in, then N x 2 scores and N desolvation
-    // terms out, in one buffer of N x 7 doubles (7 >= 2 + 1)
-    const size_t stage = (size_t)N * 7;
+    const size_t n = (size_t)N;
+    const size_t stage = n * 10;
     if (stage > c->cap_stage) {
         if (c->h_stage) (void)hipHostFree(c->h_stage);
-        c->h_stage = nullptr;
+        c->h_stage = c->d_stage = nullptr;
         c->cap_stage = 0;
-        HIPCK("alloc", hipHostMalloc((void**)&c->h_stage, sizeof(double) * stage, hipHostMallocDefault));
+        HIPCK("alloc", hipHostMalloc((void**)&c->h_stage, sizeof(double) * stage,
+                                     hipHostMallocMapped | hipHostMallocCoherent));
+        const hipError_t e = hipHostGetDevicePointer((void**)&c->d_stage, c->h_stage, 0);
+        if (e != hipSuccess || !c->d_stage) {
+            (void)hipHostFree(c->h_stage);
+            c->h_stage = c->d_stage = nullptr;
+            HIPCK("alloc", e != hipSuccess ? e : hipErrorInvalidValue);
+        }
         c->cap_stage = stage;
     }
-    memcpy(c->h_stage, poses, sizeof(double) * 7 * (size_t)N);
-    // a failure after the upload is enqueued waits for the stream first: the
-    // staging buffer is the next call's too
-    const hipError_t up = hipMemcpyAsync(c->d_poses, c->h_stage, sizeof(double) * 7 * (size_t)N,
-                                         hipMemcpyHostToDevice, c->stream);
-    if (up != hipSuccess || fail_at("upload")) (void)hipStreamSynchronize(c->stream);
-    HIPCK("upload", up);
-    if (launch_score(c, c->d_poses, N, flags, c->d_out, (flags & F_DESOLV) ? c->d_outS : nullptr,
-                     c->stream, pa_e != nullptr)) {
-        (void)hipStreamSynchronize(c->stream);
+    memcpy(c->h_stage, poses, sizeof(double) * 7 * n);
+    HIPCK("upload", hipSuccess);  // the failure-injection stage of the pose hand-over
+    double* d_res = c->d_stage + 7 * n;
+    if (launch_score(c, c->d_stage, N, flags, d_res, (flags & F_DESOLV) ? d_res + 2 * n : nullptr, c->stream,
+                     pa_e != nullptr, nullptr, nullptr, c->d_poses)) {
+        (void)hipStreamSynchronize(c->stream);  // nothing may still read the staging buffer
         return -1;
     }
-    // both copies are waited for before any failure is reported, so no DMA
-    // into the staging buffer outlives the call
-    hipError_t rb = hipMemcpyAsync(c->h_stage, c->d_out, sizeof(double) * 2 * (size_t)N,
-                                   hipMemcpyDeviceToHost, c->stream);
-    if (rb == hipSuccess && (flags & F_DESOLV))
-        rb = hipMemcpyAsync(c->h_stage + 2 * (size_t)N, c->d_outS, sizeof(double) * (size_t)N,
-                            hipMemcpyDeviceToHost, c->stream);
     HIPCK("sync", hipStreamSynchronize(c->stream));
-    HIPCK("readback", rb);
-    memcpy(out, c->h_stage, sizeof(double) * 2 * (size_t)N);
-    if (flags & F_DESOLV) memcpy(outS, c->h_stage + 2 * (size_t)N, sizeof(double) * (size_t)N);
+    HIPCK("readback", hipSuccess);
+    const double* h_res = c->h_stage + 7 * n;
+    memcpy(out, h_res, sizeof(double) * 2 * n);
+    if (flags & F_DESOLV) memcpy(outS, h_res + 2 * n, sizeof(double) * n);
     if (pa_e) {
-        std::vector<float> e((size_t)N * c->nl), v((size_t)N * c->nl);
+        std::vector<float> e(n * c->nl), v(n * c->nl);
         HIPCK("readback", hipMemcpy(e.data(), c->d_pa_e, sizeof(float) * e.size(), hipMemcpyDeviceToHost));
         HIPCK("readback", hipMemcpy(v.data(), c->d_pa_v, sizeof(float) * v.size(), hipMemcpyDeviceToHost));
         for (int k = 0; k < N; ++k)
