@@ -1140,68 +1140,83 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
     if (tid < 3) out[3 * (size_t)pose + tid] = icnt[tid];
 }
 
-// Expected near-field cost of each pose, for the dispatch order: one wavefront
-// per pose sums the neighbour-list lengths at the nodes of every eighth ligand
-// atom (the walk's work is proportional to the candidates it visits).  Each
-// lane places U atoms, then requests their offsets together.
+// Longest-first dispatch order, one launch.  xcd_tile gives XCD x the
+// contiguous range of tiles it dispatches in ascending order; within each
+// range the order sorts the poses by expected near-field cost, descending
+// (ties by pose), so the costly poses start first and the cheap ones fill the
+// last round.  The cost of a pose is the sum of the neighbour-list lengths at
+// the nodes of every eighth ligand atom (the walk's work is proportional to
+// the candidates it visits): one wavefront per pose, each lane placing U atoms
+// and requesting their offsets together.  Block b serves range b % 8 (ORDER_W
+// poses per block and round); the last block of a range to finish (counter
+// cnt[x]) reads the range's costs and writes its order by a rank sort in LDS,
+// then re-zeroes the counter for the next launch.  Blocks meet only through
+// device-scope atomics (see vm_drain), so the placement of blocks on XCDs
+// does not matter.  The wavefronts also keep a device copy of each pose
+// (pose_copy, when given): the poses may be read from host memory.  Ranges
+// of up to LPT_MAX poses.
 constexpr int COST_STRIDE = 8;
-__global__ void __launch_bounds__(64) pose_cost_kernel(DevComplex C, const double* __restrict__ poses, int N,
-                                                       unsigned* __restrict__ cost, double* __restrict__ pose_copy) {
-    const int pose = blockIdx.x;
-    if (pose >= N) return;
-    const Mat M = pose_matrix(poses + (size_t)pose * 7);
-    // poses read from host memory are kept in device memory for the scoring kernel
-    if (pose_copy && threadIdx.x < 7) pose_copy[(size_t)pose * 7 + threadIdx.x] = poses[(size_t)pose * 7 + threadIdx.x];
-    const float4* lx = C.lxyzq + (C.lstride ? (size_t)pose * (size_t)C.lstride : 0);
-    constexpr int U = 4;
-    unsigned c = 0;
-    for (int j0 = (int)threadIdx.x * COST_STRIDE; j0 < C.nl; j0 += U * 64 * COST_STRIDE) {
-        int n[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int j = j0 + u * 64 * COST_STRIDE;
-            n[u] = -1;
-            if (j < C.nl) {
-                const float4 lb = lx[j];
-                double x, y, z;
-                transform(M, lb.x, lb.y, lb.z, x, y, z);
-                n[u] = list_node(C, x, y, z);
-            }
-        }
-        unsigned lo[U], hi[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int q = n[u] >= 0 ? n[u] : 0;
-            lo[u] = C.noff[q];
-            hi[u] = C.noff[q + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) c += n[u] >= 0 ? hi[u] - lo[u] : 0u;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
-    if (threadIdx.x == 0) cost[pose] = c;
-}
-
-// Longest-first order within each XCD's range of tiles (xcd_tile gives XCD x
-// the contiguous range it dispatches in ascending order): block x sorts its
-// range by cost, descending, ties by pose (a rank sort in LDS), so the costly
-// poses start first and the cheap ones fill the last round.  Ranges of up to
-// LPT_MAX poses.
 constexpr int LPT_MAX = 2048;
-__global__ void __launch_bounds__(1024) lpt_order_kernel(const unsigned* __restrict__ cost, int N,
-                                                         int* __restrict__ order) {
+constexpr int ORDER_W = 4;  // wavefronts (poses) per block
+__global__ void __launch_bounds__(64 * ORDER_W) pose_order_kernel(DevComplex C, const double* __restrict__ poses,
+                                                                  int N, unsigned* __restrict__ cost,
+                                                                  int* __restrict__ order, unsigned* __restrict__ cnt,
+                                                                  double* __restrict__ pose_copy) {
     __shared__ unsigned long long key[LPT_MAX];
-    const unsigned q = (unsigned)N / 8u, r = (unsigned)N % 8u, x = blockIdx.x;
+    __shared__ unsigned last;
+    const unsigned q = (unsigned)N / 8u, r = (unsigned)N % 8u, x = blockIdx.x % 8u;
     const unsigned start = x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q;
     const unsigned m = q + (x < r ? 1u : 0u);
+    const unsigned nblk = gridDim.x / 8u;  // blocks per range
+    const int lane = (int)threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+    for (unsigned i = (blockIdx.x / 8u) * ORDER_W + (unsigned)w; i < m; i += nblk * ORDER_W) {
+        const unsigned pose = start + i;
+        const Mat M = pose_matrix(poses + (size_t)pose * 7);
+        if (pose_copy && lane < 7) pose_copy[(size_t)pose * 7 + lane] = poses[(size_t)pose * 7 + lane];
+        const float4* lx = C.lxyzq + (C.lstride ? (size_t)pose * (size_t)C.lstride : 0);
+        constexpr int U = 4;
+        unsigned c = 0;
+        for (int j0 = lane * COST_STRIDE; j0 < C.nl; j0 += U * 64 * COST_STRIDE) {
+            int n[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = j0 + u * 64 * COST_STRIDE;
+                n[u] = -1;
+                if (j < C.nl) {
+                    const float4 lb = lx[j];
+                    double px, py, pz;
+                    transform(M, lb.x, lb.y, lb.z, px, py, pz);
+                    n[u] = list_node(C, px, py, pz);
+                }
+            }
+            unsigned lo[U], hi[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int qn = n[u] >= 0 ? n[u] : 0;
+                lo[u] = C.noff[qn];
+                hi[u] = C.noff[qn + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) c += n[u] >= 0 ? hi[u] - lo[u] : 0u;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+        if (lane == 0) __hip_atomic_store(&cost[pose], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    vm_drain();  // this wave's cost stores are performed before the block counts itself
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(&cnt[x], 1u) == nblk - 1u ? 1u : 0u;
+    __syncthreads();
+    if (!last) return;
     for (unsigned t = threadIdx.x; t < m; t += blockDim.x) {
         // descending cost first, then ascending pose: key = ~cost : pose (unique)
-        key[t] = ((unsigned long long)(~cost[start + t]) << 32) | (start + t);
+        const unsigned ct = __hip_atomic_load(&cost[start + t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        key[t] = ((unsigned long long)(~ct) << 32) | (start + t);
     }
+    if (threadIdx.x == 0) __hip_atomic_store(&cnt[x], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     // rank sort: a key's position is the number of smaller keys (every lane
-    // reads the same key at once, an LDS broadcast; one barrier in all)
+    // reads the same key at once, an LDS broadcast)
     for (unsigned t = threadIdx.x; t < m; t += blockDim.x) {
         const unsigned long long k = key[t];
         unsigned rank = 0;
@@ -1299,6 +1314,7 @@ struct lkl_ctx {
     // per-pose cost and dispatch order of one-tile-per-pose launches
     unsigned* d_pcost = nullptr;
     int* d_porder = nullptr;
+    unsigned* d_ocnt = nullptr;  // the dispatch-order counters (8, zero between launches)
     size_t cap_pcost = 0, cap_porder = 0;
     bool pose_order = true;
     std::vector<float> lq;
@@ -1784,7 +1800,7 @@ static void ctx_free(lkl_ctx* c) {
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
-                    c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder, c->d_nidx32,
+                    c->d_noff_if, c->d_nidx_if, c->d_lcoords, c->d_pcost, c->d_porder, c->d_ocnt, c->d_nidx32,
                     c->d_nidx32_if, c->d_smin};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -1910,6 +1926,7 @@ static size_t type_lds(const lkl_ctx* c) { return 16 * (size_t)c->ntype + ((size
 struct LaunchScratch {
     unsigned* pcost = nullptr;
     int* porder = nullptr;
+    unsigned* ocnt = nullptr;
     unsigned* gmin = nullptr;
     long long* acc = nullptr;
     unsigned* cnt = nullptr;
@@ -2040,12 +2057,18 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
     if (c->pose_order && list && nchunks == 1 && !peratom && N > POSE_ORDER_MIN && (N + 7) / 8 <= LPT_MAX) {
         if (!ext && dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
         if (!ext && dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
+        if (!ext && !c->d_ocnt) {  // zeroed once; every launch leaves them zero
+            HIPCK("alloc", hipMalloc((void**)&c->d_ocnt, 8 * sizeof(unsigned)));
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, (unsigned long long*)c->d_ocnt, (size_t)4);
+        }
         unsigned* pcost = ext ? ext->pcost : c->d_pcost;
         int* porder = ext ? ext->porder : c->d_porder;
-        hipLaunchKernelGGL(pose_cost_kernel, dim3((unsigned)N), dim3(64), 0, st, C, d_poses, N, pcost, pose_copy);
-        int pm = 64;  // threads of the sort: the range rounded up to a power of two, at most 1024
-        while (pm < (N + 7) / 8 && pm < 1024) pm <<= 1;
-        hipLaunchKernelGGL(lpt_order_kernel, dim3(8), dim3((unsigned)pm), 0, st, pcost, N, porder);
+        unsigned* ocnt = ext ? ext->ocnt : c->d_ocnt;
+        if (ext)  // the shared scratch holds other launches' data: clear the counters
+            hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, st, (unsigned long long*)ocnt, (size_t)4);
+        const unsigned per_range = ((unsigned)N / 8u + 1u + ORDER_W - 1u) / ORDER_W;
+        hipLaunchKernelGGL(pose_order_kernel, dim3(8u * per_range), dim3(64 * ORDER_W), 0, st, C, d_poses, N, pcost,
+                           porder, ocnt, pose_copy);
         HIPCK("launch", hipGetLastError());
         C.porder = porder;
         c->ev_prepass = true;
@@ -2170,15 +2193,17 @@ static unsigned char* scratch_of(int dev) {
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // Scratch bytes of one no-allocation launch of N poses: cost and order (4 B
-// each), accumulators (24 B) with the counters right after them (4 B, a
-// whole number of 8-byte words), then the global desolvation minima (4 B per
-// receptor atom per pose) when they do not fit in LDS.
+// each), the 8 dispatch-order counters, accumulators (24 B) with the counters
+// right after them (4 B, a whole number of 8-byte words), then the global
+// desolvation minima (4 B per receptor atom per pose) when they do not fit in
+// LDS.
 static size_t noalloc_layout(const lkl_ctx* c, int N, unsigned flags, unsigned char* base, LaunchScratch* L) {
     size_t off = 0;
     const size_t n = (size_t)N;
     auto take = [&](size_t bytes) { const size_t o = off; off += align256(bytes); return base ? base + o : nullptr; };
     L->pcost = (unsigned*)take(4 * n);
     L->porder = (int*)take(4 * n);
+    L->ocnt = (unsigned*)take(32);
     unsigned char* a = take(24 * n + 8 * ((n + 1) / 2));
     L->acc = (long long*)a;
     L->cnt = a ? (unsigned*)(a + 24 * n) : nullptr;
