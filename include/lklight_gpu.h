@@ -166,7 +166,7 @@ int lkl_ctx_score_async_noalloc(lkl_ctx* ctx, const double* d_poses, int N, unsi
                                 double* d_out, double* d_outS, void* stream);
 
 /* Longest-first dispatch (default on): one-tile-per-pose launches of 257 to
- * 16,384 poses over neighbour lists first estimate each pose's near-field
+ * 4,096 poses over neighbour lists first estimate each pose's near-field
  * cost (list lengths at its atoms' nodes) and start the costly poses first,
  * so costly tiles share CUs with cheap ones and cheap poses fill the last
  * round of workgroup slots.  Scores are the same

@@ -86,10 +86,14 @@ enum DesMode { DES_NONE = 0, DES_LDS = 1, DES_GLOBAL = 2 };
 #define SQRT_D(x) __builtin_amdgcn_sqrtf(x)
 #endif
 constexpr int NU = LKL_NU;
-// one-tile-per-pose launches above this many poses are dispatched longest
-// first: beyond one tile per CU, tiles share CUs (and, further on, wait for a
-// free slot), and starting the costly ones first pairs them with cheap ones
+// one-tile-per-pose launches of POSE_ORDER_MIN + 1 to POSE_ORDER_MAX poses
+// are dispatched longest first: beyond one tile per CU, tiles share CUs (and,
+// further on, wait for a free slot), and starting the costly ones first pairs
+// them with cheap ones.  Above POSE_ORDER_MAX the launch runs many rounds of
+// workgroup slots, its last round is a small share of it, and the order's
+// pre-pass (a rank sort, quadratic in the range) costs more than it saves.
 constexpr int POSE_ORDER_MIN = 256;
+constexpr int POSE_ORDER_MAX = 4096;
 
 // One receptor candidate r (sorted index p) against ligand atom (x64, y64,
 // z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
@@ -1154,10 +1158,10 @@ iface_kernel(DevComplex C, IfaceDev I, const double* __restrict__ poses, int N, 
 // device-scope atomics (see vm_drain), so the placement of blocks on XCDs
 // does not matter.  The wavefronts also keep a device copy of each pose
 // (pose_copy, when given): the poses may be read from host memory.  Ranges
-// of up to LPT_MAX poses.
+// of up to LPT_MAX poses (launches of up to 8 LPT_MAX).
 constexpr int COST_STRIDE = 8;
-constexpr int LPT_MAX = 2048;
-constexpr int ORDER_W = 4;  // wavefronts (poses) per block
+constexpr int LPT_MAX = POSE_ORDER_MAX / 8;
+constexpr int ORDER_W = 16;  // wavefronts (poses) per block: 1,024 threads sort a range
 __global__ void __launch_bounds__(64 * ORDER_W) pose_order_kernel(DevComplex C, const double* __restrict__ poses,
                                                                   int N, unsigned* __restrict__ cost,
                                                                   int* __restrict__ order, unsigned* __restrict__ cnt,
@@ -2054,7 +2058,7 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         hipLaunchKernelGGL(zero_u64_kernel, dim3((unsigned)((w + 255) / 256)), dim3(256), 0, st,
                            (unsigned long long*)acc, w);
     }
-    if (c->pose_order && list && nchunks == 1 && !peratom && N > POSE_ORDER_MIN && (N + 7) / 8 <= LPT_MAX) {
+    if (c->pose_order && list && nchunks == 1 && !peratom && N > POSE_ORDER_MIN && N <= POSE_ORDER_MAX) {
         if (!ext && dev_reserve(&c->d_pcost, &c->cap_pcost, (size_t)N)) return -1;
         if (!ext && dev_reserve(&c->d_porder, &c->cap_porder, (size_t)N)) return -1;
         if (!ext && !c->d_ocnt) {  // zeroed once; every launch leaves them zero
