@@ -678,6 +678,70 @@ def gso_step_compare(cx, ctx, family, steps=50):
     return res
 
 
+def surrogate_cost_by_complex(args, net, workloads=("c1", "c2", "c3")):
+    """What a screened candidate costs against an exact evaluation on complexes
+    of increasing size: per 64,000-pose batch, the grid features and the MLP
+    forward against the exact score of the same poses (dna at c1 and c2,
+    cpydock with desolvation at c3, each with its stored tiles), and the share
+    of the plain run's steps within which a screen of 32 candidates and 2 exact
+    scores per glowworm must reach the plain top-10 for a 3x gain."""
+    import numpy as np
+    import torch
+    import lklight
+    from lklight import synth
+    from lklight import surrogate as S
+
+    nc, C, k = 64000, 32, 2
+    rows = []
+    for wl in workloads:
+        method = "cpydock" if wl == "c3" else "dna"
+        cx = synth.make_config(wl)
+        cx.build_field()
+        desolv = method == "cpydock"
+        ctx = lklight.Context(cx, desolv=desolv)
+        key = "%s-%s" % (wl, method)
+        t = lklight.tuned_tiles(key)
+        if t:  # c1 has no stored tiles: the context's defaults
+            ctx.set_tiles(t["cell"], t["block"], t["chunk"], cell_x=t["cell_x"])
+        flags = lklight.family_flags(method) | (lklight.F_DESOLV if desolv else 0)
+        t0 = time.perf_counter()
+        feats = S.Features(cx, grid=args.sur_grid)
+        t_grid = time.perf_counter() - t0
+        cand = torch.from_numpy(synth.make_poses(nc, cx.meta["rec_axes"], 5)).cuda()
+        Xc = torch.zeros(((nc + 127) // 128 * 128, S.NF), dtype=torch.int16, device="cuda")
+        yc = torch.empty(Xc.shape[0], dtype=torch.float32, device="cuda")
+        d_o = torch.zeros((nc, 2), dtype=torch.float64, device="cuda")
+        d_s = torch.zeros(nc, dtype=torch.float64, device="cuda") if desolv else None
+        tf, tm, te = [], [], []
+        for rep in range(4):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            feats.compute_device(cand, Xc[:nc], 0)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            net.forward_device(Xc, yc)
+            net.sync()
+            t2 = time.perf_counter()
+            ctx.score_async(cand.data_ptr(), nc, flags, d_o.data_ptr(), d_s.data_ptr() if desolv else None)
+            torch.cuda.synchronize()
+            t3 = time.perf_counter()
+            if rep:
+                tf.append(t1 - t0), tm.append(t2 - t1), te.append(t3 - t2)
+        f_ns, m_ns, e_ns = (float(np.median(v)) / nc * 1e9 for v in (tf, tm, te))
+        ratio = (f_ns + m_ns) / e_ns
+        rows.append({"workload": wl, "family": method, "receptor_atoms": int(cx.rec.n), "ligand_atoms": int(cx.lig.n),
+                     "grid_build_s": t_grid, "features_ns_per_candidate": f_ns, "mlp_ns_per_candidate": m_ns,
+                     "exact_ns_per_pose": e_ns, "candidate_over_exact": ratio,
+                     "screened_step_over_plain_step": C * ratio + k,
+                     "steps_share_for_3x": 1.0 / (3.0 * (C * ratio + k))})
+        log("surrogate cost at %s (%s, %d x %d atoms): features %.1f ns + MLP %.1f ns per candidate, exact %.1f ns "
+            "per pose (ratio %.2f)" % (wl, method, cx.rec.n, cx.lig.n, f_ns, m_ns, e_ns, ratio))
+        feats.close()
+        ctx.close()
+        del Xc, yc, d_o, d_s, cand
+    return rows
+
+
 def library_mlp_reference(rows, reps=10):
     """The vendor library's time for the screen's forward shapes (torch.matmul
     in bf16, which calls hipBLASLt): rows x 6144 -> 512 -> 512 -> 512 with bias
@@ -881,6 +945,7 @@ def surrogate_experiment(args):
             / (3.0 * scr_ms)},
         "train": {"poses": int(P.shape[0]), "epochs": args.sur_epochs, "data_s": t_data,
                   "train_s": t_train, "loss": hist},
+        "cost_by_complex": surrogate_cost_by_complex(args, net),
     }
     feats.close()
     net.close()
