@@ -313,7 +313,7 @@ def test_param_types_fallback(small):
 
 
 def test_pose_order_is_bitwise_neutral(small):
-    """Longest-first dispatch (launches of 257..16,384 poses) changes only the
+    """Longest-first dispatch (launches of 257..4,096 poses) changes only the
     order workgroups start in: every pose's scores are the same bits with it
     on and off, with and without desolvation."""
     cx, orc, _ = small
