@@ -263,14 +263,20 @@ int lkl_ctx_set_phi_packed(lkl_ctx* ctx, int on);
 /* 1 when the context holds the corner-packed far field, else 0. */
 int lkl_ctx_phi_packed(const lkl_ctx* ctx);
 
-/* Receptor parameter types (default on): when the receptor's per-atom
- * (sqrt(eps), vdw radius, heavy) records take at most 256 distinct values, the
- * neighbour-list walk copies them and one type byte per atom to LDS and reads
- * a candidate's parameters there instead of gathering them with its
- * coordinates.  Scores are bitwise the same either way.  Library-internal
- * layout with no counterpart in the reference (full_score.cu reads each
- * receptor atom's parameters from global memory).  lkl_ctx_param_types
- * returns the number of types in use (0 = off, or more than 256 records). */
+/* Parameter types of the list walk (level, default 2): when the receptor's
+ * per-atom (sqrt(eps), vdw radius, heavy) records take at most 256 distinct
+ * values, the neighbour-list walk copies them and one type byte per atom to
+ * LDS and reads a candidate's parameters there instead of gathering them with
+ * its coordinates (level 1).  At level 2, when the ligand's records take at
+ * most 256 values too and the table has at most 2,048 entries, each workgroup
+ * instead forms a table over (ligand type, receptor type) pairs of the LJ
+ * factor, (r_i + r_j)^2 and the clash threshold, with the walk's own f32
+ * operations, and the walk reads a pair's entry (fewer operations per
+ * candidate).  Level 0 turns the tables off.  Scores are bitwise the same at
+ * every level.  Library-internal layout with no counterpart in the reference
+ * (full_score.cu reads each receptor atom's parameters from global memory).
+ * lkl_ctx_param_types returns the number of receptor types in use (0 = off,
+ * or more than 256 records). */
 int lkl_ctx_set_param_types(lkl_ctx* ctx, int on);
 int lkl_ctx_param_types(const lkl_ctx* ctx);
 

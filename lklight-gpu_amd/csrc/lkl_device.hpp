@@ -93,6 +93,13 @@ struct DevComplex {
     const unsigned* rtyp;
     const float4* rtab;
     int ntype;
+    // ligand parameter types (NULL = none): the distinct lpar records (at most
+    // 256) and one type byte per sorted ligand atom, zero-padded to whole
+    // words; with the receptor's types the list walk may read each pair's LJ
+    // and clash factors from an LDS table over (ligand type, receptor type)
+    const unsigned char* lty;
+    const float4* lttab;
+    int nltype;
     int nnx, nny, nnz;
     float nox, noy, noz, inv_nh;
     // ligand, spatially sorted (Morton order of the reference coordinates)

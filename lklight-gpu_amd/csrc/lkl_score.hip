@@ -101,7 +101,21 @@ constexpr int POSE_ORDER_MAX = 4096;
 // cpydock.rs:620-627.  (xh + xl) is the hi/lo f32 split of the f64 coordinate.
 // With PRE the caller has already loaded the candidate's parameters (rp_pre);
 // otherwise they are loaded for the pairs within the cutoff only.
-template <bool ELEC, bool CLASH, int DES, bool PRE = false>
+// The pair-type entry of a ligand record lp and a receptor record rp: the
+// walk's LJ factor sqrt(eps_j) V_SCALE sqrt(eps_i), (r_i + r_j)^2, the clash
+// threshold 0.75 (r_i + r_j) when both atoms are heavy (else 0: no clash) and
+// r_i + r_j, each formed by the f32 operations pair_term applies to the two
+// records, so a walk over these entries gives the same bits.
+__device__ __forceinline__ float4 pair_entry(float4 lp, float4 rp) {
+    const float svjs = lp.x * V_SCALE_F;
+    const float rr = lp.y + fabsf(rp.y);
+    const bool clash = lp.z != 0.0f && rp.z != 0.0f;
+    return make_float4(svjs * rp.x, rr * rr, clash ? CP_F * rr : 0.0f, rr);
+}
+
+// With PAIR the preloaded rp_pre is the pair-type entry (pair_entry) instead
+// of the receptor record (no desolvation).
+template <bool ELEC, bool CLASH, int DES, bool PRE = false, bool PAIR = false>
 __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, int p, double x64, double y64,
                                           double z64, float xh, float yh, float zh, float xl, float yl,
                                           float zl, float qjs, float svjs, float rj, bool hj, bool lf,
@@ -128,18 +142,30 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
             const float ae = qjs * r.w * inv;
             e += min(max(__float2int_rn(ae), -(int)ES_CAP_I), (int)ES_CAP_I);
         }
-        const float svs = svjs * rp.x;
-        const float rr = rj + fabsf(rp.y);
-        const float t = (rr * rr) * inv;  // (r_i + r_j)^2 / d^2
-        const float p6 = t * t * t;
-        float vp = svs * (p6 * p6 - 2.0f * p6);
-        vp = fminf(vp, LJ_CAP * V_SCALE_F);
-        if (CLASH && hj && rp.z != 0.0f) {
-            const float d = SQRT_D(d2);
-            const float dmin = CP_F * rr;
-            if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+        if constexpr (PAIR) {  // rp: (LJ factor, (r_i + r_j)^2, clash threshold or 0, r_i + r_j)
+            const float t = rp.y * inv;
+            const float p6 = t * t * t;
+            float vp = rp.x * (p6 * p6 - 2.0f * p6);
+            vp = fminf(vp, LJ_CAP * V_SCALE_F);
+            if (CLASH) {
+                const float d = SQRT_D(d2);
+                if (d < rp.z) vp += (CP_W * V_SCALE_F) * (CP_F * rp.w - d);
+            }
+            v += (long long)__float2int_rn(vp);
+        } else {
+            const float svs = svjs * rp.x;
+            const float rr = rj + fabsf(rp.y);
+            const float t = (rr * rr) * inv;  // (r_i + r_j)^2 / d^2
+            const float p6 = t * t * t;
+            float vp = svs * (p6 * p6 - 2.0f * p6);
+            vp = fminf(vp, LJ_CAP * V_SCALE_F);
+            if (CLASH && hj && rp.z != 0.0f) {
+                const float d = SQRT_D(d2);
+                const float dmin = CP_F * rr;
+                if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+            }
+            v += (long long)__float2int_rn(vp);
         }
-        v += (long long)__float2int_rn(vp);
     }
 }
 
@@ -293,6 +319,7 @@ constexpr float WORK_H = 2.0f, WORK_PAD = 12.0f;
 constexpr int WORK_TILE_MAX = 16384;  // ligand atoms per work-ordered tile (LDS arrays)
 constexpr unsigned IFACE_PAD = 4;     // entries per padding unit of the interface pass's lists
 constexpr int RTYPE_MAX = 256;        // receptor parameter types of the LDS table (one byte each)
+constexpr int PAIR_TAB_MAX = 2048;    // entries of the pair-type LDS table (32 KB)
 
 __global__ void workgrid_kernel(const float4* __restrict__ rec, int nr, float ox, float oy, float oz,
                                 int nx, int ny, long long nodes, float r2, unsigned short* __restrict__ out) {
@@ -531,7 +558,7 @@ constexpr int GA = 64 / GL;  // atoms per group (one wavefront)
 constexpr int GK = 8;        // entries per lane per step
 constexpr int GB = 4;        // step-count buckets of the active atoms
 
-template <bool FAR, bool ELEC, bool CLASH, int DES, bool TYPED = false, bool WIDE = false>
+template <bool FAR, bool ELEC, bool CLASH, int DES, int TYPED = 0, bool WIDE = false>
 __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                             unsigned* __restrict__ minR, long long& e, long long& v,
                                             long long& s) {
@@ -543,9 +570,16 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
     unsigned* ae = ab + nT;
     unsigned short* act = reinterpret_cast<unsigned short*>(ae + nT);
     float4* ltab = reinterpret_cast<float4*>(reinterpret_cast<char*>(lds) + ((10u * (unsigned)nT + 15u) & ~15u));
-    unsigned* ltyp = reinterpret_cast<unsigned*>(ltab + (TYPED ? C.ntype : 0));
-    if constexpr (TYPED) {  // visible to the walk after pass 1's barrier
-        for (int k = tid; k < C.ntype; k += bs) ltab[k] = C.rtab[k];
+    // TYPED 1: the receptor's type table; 2: the pair-type table, row
+    // ligand type, column receptor type (pair_entry)
+    const int ntab = TYPED == 2 ? C.nltype * C.ntype : TYPED ? C.ntype : 0;
+    unsigned* ltyp = reinterpret_cast<unsigned*>(ltab + ntab);
+    if constexpr (TYPED != 0) {  // visible to the walk after pass 1's barrier
+        if constexpr (TYPED == 2) {
+            for (int k = tid; k < ntab; k += bs) ltab[k] = pair_entry(C.lttab[k / C.ntype], C.rtab[k % C.ntype]);
+        } else {
+            for (int k = tid; k < C.ntype; k += bs) ltab[k] = C.rtab[k];
+        }
         const int nwd = (C.nr + 4) >> 2;  // (nr + 1) type bytes in whole words
         for (int k = tid; k < nwd; k += bs) ltyp[k] = C.rtyp[k];
     }
@@ -680,6 +714,7 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
         const bool hj = lp.z != 0.0f;
         const float xh = (float)x, yh = (float)y, zh = (float)z;
         const float xl = (float)(x - (double)xh), yl = (float)(y - (double)yh), zl = (float)(z - (double)zh);
+        const float4* prow = TYPED == 2 ? ltab + (int)C.lty[j] * C.ntype : ltab;  // this atom's row of pairs
         double mloc = MIN_INIT;
         // a lane's GK indices of a step are one 16-byte load from the
         // lane-major block (the dummy block once its list has ended); the next
@@ -718,15 +753,15 @@ __device__ __forceinline__ void near_groups(const DevComplex& C, const Mat& M, i
                 // shift per record; both halves of a record share it).
                 const unsigned off = (unsigned)id[k] << (TYPED ? 4 : 5);
                 rq[k] = *reinterpret_cast<const float4*>(rbase + off);
-                if constexpr (TYPED) rp[k] = ltab[reinterpret_cast<const unsigned char*>(ltyp)[id[k]]];
+                if constexpr (TYPED != 0) rp[k] = prow[reinterpret_cast<const unsigned char*>(ltyp)[id[k]]];
                 else rp[k] = *reinterpret_cast<const float4*>(rbase + off + 16u);
             }
             int idn[GK];
             step_ids(p + (unsigned)(GK * GL), idn);
 #pragma unroll
             for (int k = 0; k < GK; ++k)
-                pair_term<ELEC, CLASH, DES, true>(C, rq[k], id[k], x, y, z, xh, yh, zh, xl, yl, zl, qjs, svjs, rj,
-                                                  hj, lf, e, v, mloc, minR, rp[k]);
+                pair_term<ELEC, CLASH, DES, true, TYPED == 2>(C, rq[k], id[k], x, y, z, xh, yh, zh, xl, yl, zl, qjs,
+                                                              svjs, rj, hj, lf, e, v, mloc, minR, rp[k]);
 #pragma unroll
             for (int k = 0; k < GK; ++k) id[k] = idn[k];
         }
@@ -768,7 +803,7 @@ __device__ __forceinline__ void vm_drain() { __builtin_amdgcn_s_waitcnt(0x0F70);
 // Launch bounds of 1024 threads also cap every variant at 128 VGPRs, so two
 // 512-thread tiles always share a CU (the desolvation variants would otherwise
 // take up to 148 and run one tile per CU).
-template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM, bool TYPED = false>
+template <bool LIST, bool FAR, bool ELEC, bool CLASH, int DES, bool PERATOM, int TYPED = 0>
 __global__ void __launch_bounds__(1024) LKL_SCORE_ATTR
 score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __restrict__ out,
              double* __restrict__ outS, unsigned* __restrict__ gmin, float* __restrict__ pa_e,
@@ -944,7 +979,7 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
 // Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
 // thread's share (no desolvation, no per-atom output).  A problem whose view
 // carries parameter types (ntype > 0; the host clears it when the table does
-// not fit the launch's LDS) walks its lists with them.
+// not fit the launch's LDS) walks its lists with the receptor's type table.
 template <bool LIST, bool FAR, bool ELEC, bool CLASH>
 __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
@@ -952,11 +987,11 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
         long long s_unused = 0;
         if constexpr (LIST) {
             if (C.nidx32) {
-                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
-                else near_groups<FAR, ELEC, CLASH, DES_NONE, false, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else near_groups<FAR, ELEC, CLASH, DES_NONE, 0, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
             } else {
-                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
-                else near_groups<FAR, ELEC, CLASH, DES_NONE, false>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else near_groups<FAR, ELEC, CLASH, DES_NONE, 0>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
             }
         } else
             near_sorted<false, FAR, ELEC, CLASH, DES_NONE>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
@@ -1283,7 +1318,13 @@ struct lkl_ctx {
     unsigned* d_rtyp = nullptr;
     float4* d_rtab = nullptr;
     int ntype = 0;
-    bool param_types = true;  // the list walk may read parameters from the LDS type table
+    // ligand parameter types (nltype 0: more than RTYPE_MAX distinct records)
+    unsigned* d_lty = nullptr;
+    float4* d_lttab = nullptr;
+    int nltype = 0;
+    // LDS type tables of the list walk: 0 none, 1 the receptor's, 2 (when the
+    // ligand has types and the table fits) the pair types
+    int param_types = 2;
     int* d_cstart = nullptr;
     unsigned char* d_rflag = nullptr;
     float *d_rdes = nullptr, *d_rasa = nullptr;
@@ -1717,6 +1758,32 @@ static int build_ligand(lkl_ctx* c, const float* xyz, const float* ele, const fl
     for (int s = 0; s < nl; ++s) c->lq[s] = lx[s].w;
     if (dev_upload(&c->d_lxyzq, lx.data(), (size_t)nl, c->stream)) return -1;
     if (dev_upload(&c->d_lpar, lp.data(), (size_t)nl, c->stream)) return -1;
+    {
+        // ligand parameter types: the distinct lpar records (sorted order),
+        // when there are at most RTYPE_MAX of them
+        std::map<std::array<unsigned, 3>, int> ids;
+        std::vector<float4> tab;
+        std::vector<unsigned char> typ(((size_t)nl + 3) / 4 * 4, 0);
+        bool fits = true;
+        for (int j = 0; j < nl && fits; ++j) {
+            std::array<unsigned, 3> key;
+            std::memcpy(key.data(), &lp[j], sizeof key);
+            auto it = ids.find(key);
+            if (it == ids.end()) {
+                if ((int)tab.size() == RTYPE_MAX) { fits = false; break; }
+                it = ids.emplace(key, (int)tab.size()).first;
+                tab.push_back(lp[j]);
+            }
+            typ[j] = (unsigned char)it->second;
+        }
+        c->nltype = 0;
+        if (fits && !tab.empty()) {
+            if (dev_upload(&c->d_lttab, tab.data(), tab.size(), c->stream)) return -1;
+            if (dev_upload(&c->d_lty, reinterpret_cast<const unsigned*>(typ.data()), typ.size() / 4, c->stream))
+                return -1;
+            c->nltype = (int)tab.size();
+        }
+    }
     if (flag) {
         if (dev_upload(&c->d_lflag, lf.data(), (size_t)nl, c->stream)) return -1;
         if (dev_upload(&c->d_ldes, ld.data(), (size_t)nl, c->stream)) return -1;
@@ -1736,6 +1803,11 @@ static DevComplex dev_view(const lkl_ctx* c) {
         C.rtyp = c->d_rtyp;
         C.rtab = c->d_rtab;
         C.ntype = c->ntype;
+    }
+    if (c->nltype) {
+        C.lty = reinterpret_cast<const unsigned char*>(c->d_lty);
+        C.lttab = c->d_lttab;
+        C.nltype = c->nltype;
     }
     C.cstart = c->d_cstart;
     C.rflag = c->d_rflag;
@@ -1800,7 +1872,7 @@ static DevComplex dev_view(const lkl_ctx* c) {
 static void ctx_free(lkl_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_rtyp, c->d_rtab, c->d_cstart, c->d_rflag, c->d_rdes,
+    void* bufs[] = {c->d_phi, c->d_phi8, c->d_rxyzq, c->d_rpar, c->d_rrec, c->d_rtyp, c->d_rtab, c->d_lty, c->d_lttab, c->d_cstart, c->d_rflag, c->d_rdes,
                     c->d_rasa, c->d_lxyzq, c->d_lpar, c->d_lflag, c->d_ldes, c->d_lasa,
                     c->d_poses, c->d_out, c->d_outS, c->d_gmin, c->d_pa_e, c->d_pa_v, c->d_xyz,
                     c->d_acc, c->d_cnt, c->d_wgrid, c->d_noff, c->d_nidx, c->d_iface, c->d_icounts,
@@ -1894,7 +1966,7 @@ static int reserve_acc(lkl_ctx* c, size_t N) {
     return 0;
 }
 
-template <bool LIST, int DES, bool PA, bool TY = false>
+template <bool LIST, int DES, bool PA, int TY = 0>
 static score_fn pick3(unsigned f) {
     switch (f & 7u) {
         case 0: return score_kernel<LIST, false, false, false, DES, PA, TY>;
@@ -1922,6 +1994,12 @@ static size_t tile_lds(size_t nT, bool grouped) { return grouped ? 10 * nT + 16 
 // LDS bytes of the receptor's parameter types beside a grouped tile: the
 // table and one type byte per sorted atom and the dummy, in whole words.
 static size_t type_lds(const lkl_ctx* c) { return 16 * (size_t)c->ntype + ((size_t)c->nr + 4) / 4 * 4; }
+// LDS of the pair-type table and the receptor's type bytes; 0 when the
+// context cannot use it (no ligand types, or a table over PAIR_TAB_MAX)
+static size_t pair_lds(const lkl_ctx* c) {
+    if (!c->ntype || !c->nltype || c->ntype * c->nltype > PAIR_TAB_MAX) return 0;
+    return 16 * (size_t)c->ntype * (size_t)c->nltype + ((size_t)c->nr + 4) / 4 * 4;
+}
 
 // Device scratch of one launch given by the caller instead of the context's
 // own grown-on-demand buffers: per-pose cost and dispatch order, desolvation
@@ -2030,9 +2108,12 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         shm = tile_lds((size_t)chunk, list);
         // the list walk reads receptor parameters from an LDS type table when
         // the receptor has one and the table fits beside the tile's arrays
-        const size_t typed = shm + type_lds(c);
-        if (list && !peratom && c->param_types && c->ntype && typed + 1024 <= c->lds_max) {
-            fn = pick3<true, DES_NONE, false, true>(flags);
+        const size_t typed = shm + type_lds(c), paired = shm + pair_lds(c);
+        if (list && !peratom && c->param_types >= 2 && pair_lds(c) && paired + 1024 <= c->lds_max) {
+            fn = pick3<true, DES_NONE, false, 2>(flags);
+            shm = paired;
+        } else if (list && !peratom && c->param_types && c->ntype && typed + 1024 <= c->lds_max) {
+            fn = pick3<true, DES_NONE, false, 1>(flags);
             shm = typed;
         }
     }
@@ -2516,7 +2597,7 @@ LKL_API int lkl_ctx_phi_packed(const lkl_ctx* c) { return c && c->d_phi8 ? 1 : 0
 
 LKL_API int lkl_ctx_set_param_types(lkl_ctx* c, int on) {
     if (!c) { set_err("lkl_ctx_set_param_types: null context"); return -1; }
-    c->param_types = on != 0;
+    c->param_types = on <= 0 ? 0 : on == 1 ? 1 : 2;
     return 0;
 }
 
@@ -2710,7 +2791,9 @@ static int batch_refresh(lkl_multi* b) {
             const size_t tmax = std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (b->list ? 10 : 3));
             chunk = std::min(chunk, (int)tmax);
             size_t need = tile_lds((size_t)chunk, b->list);
-            // the parameter-type table beside the tile's arrays, when it fits
+            // the receptor's parameter-type table beside the tile's arrays,
+            // when it fits (the batch does not use the pair-type tables: their
+            // variants made the batched kernel slower)
             if (b->list && c->param_types && c->ntype && need + type_lds(c) + 1024 <= c->lds_max)
                 need += type_lds(c);
             else

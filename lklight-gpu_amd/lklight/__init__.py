@@ -255,9 +255,10 @@ class Context:
     def phi_packed(self):
         return bool(lib().lkl_ctx_phi_packed(self.h))
 
-    def set_param_types(self, on):
-        """lkl_ctx_set_param_types: read receptor parameters from the LDS type table."""
-        _capi.check(lib().lkl_ctx_set_param_types(self.h, int(bool(on))), "lkl_ctx_set_param_types")
+    def set_param_types(self, level):
+        """lkl_ctx_set_param_types: LDS type tables of the list walk (0 none,
+        1 the receptor's, 2 pair types when the ligand has types too)."""
+        _capi.check(lib().lkl_ctx_set_param_types(self.h, int(level)), "lkl_ctx_set_param_types")
 
     def param_types(self):
         """Receptor parameter types in use by the list walk (0 = none)."""

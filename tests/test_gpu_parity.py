@@ -115,7 +115,7 @@ def test_tile_and_order_invariance(small):
              # table and LDS desolvation minima, then the cell-row walk
              (7.5, 1024, 0, 1.25, 1.0), (5.0, 1024, 256, 0.0, 0.0)]):
         packed = i % 2 == 0
-        types = i % 4 < 2
+        types = (2, 1, 0, 2, 1)[i % 5]  # pair types, receptor types, none
         say(f"  cell {cell} block {block} chunk {chunk} cell_x {cell_x} list spacing {lh} packed phi {packed}"
             f" parameter types {types}")
         ctx.set_tiles(cell, block, chunk, cell_x=cell_x)
