@@ -520,9 +520,13 @@ def test_config2_full_size(c2, method):
 
 
 def test_blocking_call_equals_async_and_times_kernels(c2):
-    """The blocking host-buffer call (pinned staging, one synchronisation) and
-    the asynchronous call on device buffers give the same bits, and the
-    context's kernel timing reports both kernels of a longest-first launch."""
+    """The blocking host-buffer call (a pinned, device-mapped staging buffer
+    that the kernels read the poses from and write the scores into; one
+    synchronisation) and the asynchronous call on device buffers give the same
+    bits, and the context's kernel timing reports both kernels of a
+    longest-first launch.  Calls without the dispatch-order pre-pass (17 poses,
+    and 5,000: above its range) read the staging buffer from the scoring
+    kernel itself."""
     import torch
     cx, orc = c2
     poses = synth.make_poses(1200, cx.meta["rec_axes"], 11)
@@ -540,6 +544,10 @@ def test_blocking_call_equals_async_and_times_kernels(c2):
     # a smaller call after a larger one reuses the staging buffer
     e2, v2, _ = ctx.score(poses[:17], lklight.FLAGS_DNA)
     assert np.array_equal(e2, e[:17]) and np.array_equal(v2, v[:17])
+    # a call above the ordered range grows it and keeps the same bits
+    big = np.concatenate([poses] * 4 + [poses[:200]])
+    e3, v3, _ = ctx.score(big, lklight.FLAGS_DNA)
+    assert np.array_equal(e3[:1200], e) and np.array_equal(v3[3600:4800], v) and np.array_equal(e3[4800:], e[:200])
     ctx.close()
 
 
