@@ -2109,7 +2109,10 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         // the list walk reads receptor parameters from an LDS type table when
         // the receptor has one and the table fits beside the tile's arrays
         const size_t typed = shm + type_lds(c), paired = shm + pair_lds(c);
-        if (list && !peratom && c->param_types >= 2 && pair_lds(c) && paired + 1024 <= c->lds_max) {
+        // the pair-type table for the families without the clash term: the
+        // clash variant over it spills at the register cap and ran no faster
+        if (list && !peratom && !(flags & F_CLASH) && c->param_types >= 2 && pair_lds(c) &&
+            paired + 1024 <= c->lds_max) {
             fn = pick3<true, DES_NONE, false, 2>(flags);
             shm = paired;
         } else if (list && !peratom && c->param_types && c->ntype && typed + 1024 <= c->lds_max) {
