@@ -1205,16 +1205,13 @@ LKL_API int lkl_mlp_get_grads(lkl_mlp* n, float* out) {
 }
 
 // C = op(A) op(B) with an epilogue.  K-contiguous operands take gemm2_kernel
-// (split over K when the tile count is small; ws holds the partials), except
-// short products that would be split; other strides the general gemm_kernel.
+// (split over K when the tile count is small; ws holds the partials),
+// other strides the general gemm_kernel.
 template <int EPI>
 static void gemm(hipStream_t st, const bf16* A, long sam, long sak, const bf16* B, long sbk, long sbn,
                  int M, int N, int K, float* C32, bf16* C16, long ldc, const float* bias,
                  const bf16* mask, float* ws = nullptr) {
-    // short K-contiguous products that would be split take the 128 x 128
-    // tiles of gemm_kernel whole: no fp32 partials, no reduction pass
-    const bool short_k = K <= 1024 && M % TM == 0 && N % TN == 0 && K % TK == 0;
-    if (k_contiguous(sak, sbk, M, N, K) && !(short_k && split_of(M, N, K) > 1)) {
+    if (k_contiguous(sak, sbk, M, N, K)) {
         const int S = split_of(M, N, K);
         Gemm2Args a = {A, B, sam, sbn, M, N, K, K / S, S > 1 ? ws : nullptr, C32, C16, ldc, bias, mask};
         const unsigned tiles = (unsigned)(((M + BM2 - 1) / BM2) * (N / BN2));
