@@ -136,7 +136,10 @@ lkl_ctx* lkl_ctx_create(const lkl_complex_desc* desc);
 void lkl_ctx_destroy(lkl_ctx* ctx);
 
 /* Blocking, host buffers: poses N*7, out N*2 (E_raw, V), outS N (needed only
- * with LKL_F_DESOLV, may be NULL otherwise). */
+ * with LKL_F_DESOLV, may be NULL otherwise).  The poses are copied into the
+ * context's pinned, device-mapped staging buffer, which the kernels read over
+ * the bus and the scoring kernel writes the scores into (no DMA copies, one
+ * stream synchronisation).  One call at a time per context. */
 int lkl_ctx_score(lkl_ctx* ctx, const double* poses, int N, unsigned flags, double* out,
                   double* outS);
 
