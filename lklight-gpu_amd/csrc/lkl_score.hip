@@ -95,12 +95,6 @@ constexpr int NU = LKL_NU;
 constexpr int POSE_ORDER_MIN = 256;
 constexpr int POSE_ORDER_MAX = 4096;
 
-// One receptor candidate r (sorted index p) against ligand atom (x64, y64,
-// z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
-// directly in fixed-point units, and (with desolvation) the per-atom minima of
-// cpydock.rs:620-627.  (xh + xl) is the hi/lo f32 split of the f64 coordinate.
-// With PRE the caller has already loaded the candidate's parameters (rp_pre);
-// otherwise they are loaded for the pairs within the cutoff only.
 // The pair-type entry of a ligand record lp and a receptor record rp: the
 // walk's LJ factor sqrt(eps_j) V_SCALE sqrt(eps_i), (r_i + r_j)^2, the clash
 // threshold 0.75 (r_i + r_j) when both atoms are heavy (else 0: no clash) and
@@ -113,8 +107,14 @@ __device__ __forceinline__ float4 pair_entry(float4 lp, float4 rp) {
     return make_float4(svjs * rp.x, rr * rr, clash ? CP_F * rr : 0.0f, rr);
 }
 
-// With PAIR the preloaded rp_pre is the pair-type entry (pair_entry) instead
-// of the receptor record (no desolvation).
+// One receptor candidate r (sorted index p) against ligand atom (x64, y64,
+// z64): the pair terms of full_score.cu:296-315 / dna.rs:951-969 formed
+// directly in fixed-point units, and (with desolvation) the per-atom minima of
+// cpydock.rs:620-627.  (xh + xl) is the hi/lo f32 split of the f64 coordinate.
+// With PRE the caller has already loaded the candidate's parameters (rp_pre);
+// otherwise they are loaded for the pairs within the cutoff only.  With PAIR
+// the preloaded rp_pre is the pair-type entry (pair_entry) instead of the
+// receptor record (no desolvation).
 template <bool ELEC, bool CLASH, int DES, bool PRE = false, bool PAIR = false>
 __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, int p, double x64, double y64,
                                           double z64, float xh, float yh, float zh, float xl, float yl,
@@ -1378,9 +1378,10 @@ struct lkl_ctx {
     long long* d_acc = nullptr;  // per-pose tile accumulators (kept zero between launches)
     unsigned* d_cnt = nullptr;
     size_t cap_acc = 0;
-    // pinned host staging of the blocking call (poses in, scores out): one
-    // DMA each way and a single stream synchronisation per call
-    double* h_stage = nullptr;  // pinned, device-mapped staging of the host-buffer call
+    // pinned, device-mapped staging of the blocking call (poses in, scores
+    // out), read and written by the kernels over the bus: no DMA copies and a
+    // single stream synchronisation per call
+    double* h_stage = nullptr;
     double* d_stage = nullptr;  // its device address
     size_t cap_stage = 0;
     // optional kernel timing of every launch: events before the dispatch-order
