@@ -275,7 +275,7 @@ int lkl_ctx_phi_packed(const lkl_ctx* ctx);
  * instead forms a table over (ligand type, receptor type) pairs of the LJ
  * factor, (r_i + r_j)^2 and the clash threshold, with the walk's own f32
  * operations, and the walk reads a pair's entry (fewer operations per
- * candidate); launches with the clash term keep the receptor's table.  Level 0 turns the tables off.  Scores are bitwise the same at
+ * candidate).  Level 0 turns the tables off.  Scores are bitwise the same at
  * every level.  Library-internal layout with no counterpart in the reference
  * (full_score.cu reads each receptor atom's parameters from global memory).
  * lkl_ctx_param_types returns the number of receptor types in use (0 = off,

@@ -96,15 +96,23 @@ constexpr int POSE_ORDER_MIN = 256;
 constexpr int POSE_ORDER_MAX = 4096;
 
 // The pair-type entry of a ligand record lp and a receptor record rp: the
-// walk's LJ factor sqrt(eps_j) V_SCALE sqrt(eps_i), (r_i + r_j)^2, the clash
-// threshold 0.75 (r_i + r_j) when both atoms are heavy (else 0: no clash) and
-// r_i + r_j, each formed by the f32 operations pair_term applies to the two
-// records, so a walk over these entries gives the same bits.
+// walk's LJ factor sqrt(eps_j) V_SCALE sqrt(eps_i), (r_i + r_j)^2 and the clash
+// threshold 0.75 (r_i + r_j) when both atoms are heavy (else 0: no clash),
+// each formed by the f32 operations pair_term applies to the two records, so a
+// walk over these entries gives the same bits.
 __device__ __forceinline__ float4 pair_entry(float4 lp, float4 rp) {
     const float svjs = lp.x * V_SCALE_F;
     const float rr = lp.y + fabsf(rp.y);
     const bool clash = lp.z != 0.0f && rp.z != 0.0f;
-    return make_float4(svjs * rp.x, rr * rr, clash ? CP_F * rr : 0.0f, rr);
+    return make_float4(svjs * rp.x, rr * rr, clash ? CP_F * rr : 0.0f, 0.0f);
+}
+
+// The clash gap dmin - d, rounded on its own: not fused with the product that
+// forms dmin, so a walk that reads dmin from the pair-type table and one that
+// forms it from the two radii give the same bits.
+__device__ __forceinline__ float clash_gap(float dmin, float d) {
+#pragma clang fp contract(off)
+    return dmin - d;
 }
 
 // One receptor candidate r (sorted index p) against ligand atom (x64, y64,
@@ -142,14 +150,14 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
             const float ae = qjs * r.w * inv;
             e += min(max(__float2int_rn(ae), -(int)ES_CAP_I), (int)ES_CAP_I);
         }
-        if constexpr (PAIR) {  // rp: (LJ factor, (r_i + r_j)^2, clash threshold or 0, r_i + r_j)
+        if constexpr (PAIR) {  // rp: (LJ factor, (r_i + r_j)^2, clash threshold or 0, unused)
             const float t = rp.y * inv;
             const float p6 = t * t * t;
             float vp = rp.x * (p6 * p6 - 2.0f * p6);
             vp = fminf(vp, LJ_CAP * V_SCALE_F);
             if (CLASH) {
                 const float d = SQRT_D(d2);
-                if (d < rp.z) vp += (CP_W * V_SCALE_F) * (CP_F * rp.w - d);
+                if (d < rp.z) vp += (CP_W * V_SCALE_F) * clash_gap(rp.z, d);
             }
             v += (long long)__float2int_rn(vp);
         } else {
@@ -162,7 +170,7 @@ __device__ __forceinline__ void pair_term(const DevComplex& C, const float4 r, i
             if (CLASH && hj && rp.z != 0.0f) {
                 const float d = SQRT_D(d2);
                 const float dmin = CP_F * rr;
-                if (d < dmin) vp += (CP_W * V_SCALE_F) * (dmin - d);
+                if (d < dmin) vp += (CP_W * V_SCALE_F) * clash_gap(dmin, d);
             }
             v += (long long)__float2int_rn(vp);
         }
@@ -2109,10 +2117,9 @@ static int launch_score(lkl_ctx* c, const double* d_poses, int N, unsigned flags
         shm = tile_lds((size_t)chunk, list);
         // the list walk reads receptor parameters from an LDS type table when
         // the receptor has one and the table fits beside the tile's arrays
+        // (ligand x receptor pair types first, when they fit)
         const size_t typed = shm + type_lds(c), paired = shm + pair_lds(c);
-        // the pair-type table for the families without the clash term: the
-        // clash variant over it spills at the register cap and ran no faster
-        if (list && !peratom && !(flags & F_CLASH) && c->param_types >= 2 && pair_lds(c) &&
+        if (list && !peratom && c->param_types >= 2 && pair_lds(c) &&
             paired + 1024 <= c->lds_max) {
             fn = pick3<true, DES_NONE, false, 2>(flags);
             shm = paired;
