@@ -317,14 +317,17 @@ def test_pose_order_is_bitwise_neutral(small):
     order workgroups start in: every pose's scores are the same bits with it
     on and off, with and without desolvation."""
     cx, orc, _ = small
-    poses = synth.make_poses(2600, cx.meta["rec_axes"], 77)
-    other = synth.make_poses(2600, cx.meta["rec_axes"], 78)
+    poses = synth.make_poses(4096, cx.meta["rec_axes"], 77)
+    other = synth.make_poses(4096, cx.meta["rec_axes"], 78)
     ctx = lklight.Context(cx, desolv=True)
     # Between the reference and the ordered launch the device buffers hold
     # another pose set's results, so a pose the order left out would show
-    # them (300 poses: below the pre-pass threshold, the same launch).
+    # them (300 poses: below the pre-pass threshold, the same launch; 257
+    # and 4,096: the smallest and largest ordered launches, whose per-XCD
+    # ranges are 32-33 and exactly 512 poses).
     for flags, n in ((lklight.family_flags("dna"), 1500), (lklight.family_flags("dna"), 2600),
-                     (lklight.family_flags("dna"), 300),
+                     (lklight.family_flags("dna"), 300), (lklight.family_flags("pydock"), 257),
+                     (lklight.family_flags("vdw"), 4096),
                      (lklight.F_FAR | lklight.F_ELEC | lklight.F_DESOLV, 700)):
         ctx.set_pose_order(0)
         ref = run(ctx, poses[:n], flags)
