@@ -986,8 +986,9 @@ score_kernel(DevComplex C, const double* __restrict__ poses, int N, double* __re
 // ── multi-problem batch ───────────────────────────────────────────────────────
 // Fixed-point (E, V) sums of ligand atoms [j0, j1) of one pose over this
 // thread's share (no desolvation, no per-atom output).  A problem whose view
-// carries parameter types (ntype > 0; the host clears it when the table does
-// not fit the launch's LDS) walks its lists with the receptor's type table.
+// carries ligand and receptor parameter types walks its lists with the
+// pair-type table, one with receptor types only with the receptor's table
+// (the host clears the types that do not fit the launch's LDS).
 template <bool LIST, bool FAR, bool ELEC, bool CLASH>
 __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int j0, int j1, void* lds,
                                           long long& e, long long& v) {
@@ -995,10 +996,12 @@ __device__ __forceinline__ void tile_sums(const DevComplex& C, const Mat& M, int
         long long s_unused = 0;
         if constexpr (LIST) {
             if (C.nidx32) {
-                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                if (C.ntype && C.nltype) near_groups<FAR, ELEC, CLASH, DES_NONE, 2, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
                 else near_groups<FAR, ELEC, CLASH, DES_NONE, 0, true>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
             } else {
-                if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                if (C.ntype && C.nltype) near_groups<FAR, ELEC, CLASH, DES_NONE, 2>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
+                else if (C.ntype) near_groups<FAR, ELEC, CLASH, DES_NONE, 1>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
                 else near_groups<FAR, ELEC, CLASH, DES_NONE, 0>(C, M, j0, j1, lds, nullptr, e, v, s_unused);
             }
         } else
@@ -2802,16 +2805,21 @@ static int batch_refresh(lkl_multi* b) {
             const size_t tmax = std::min<size_t>(WORK_TILE_MAX, (c->lds_max - 2048) / (b->list ? 10 : 3));
             chunk = std::min(chunk, (int)tmax);
             size_t need = tile_lds((size_t)chunk, b->list);
-            // the receptor's parameter-type table beside the tile's arrays,
-            // when it fits (the batch does not use the pair-type tables: their
-            // variants made the batched kernel slower)
-            if (b->list && c->param_types && c->ntype && need + type_lds(c) + 1024 <= c->lds_max)
+            // the pair-type table, else the receptor's parameter-type table,
+            // beside the tile's arrays, when it fits
+            if (b->list && c->param_types >= 2 && pair_lds(c) && need + pair_lds(c) + 1024 <= c->lds_max) {
+                need += pair_lds(c);
+            } else if (b->list && c->param_types && c->ntype && need + type_lds(c) + 1024 <= c->lds_max) {
                 need += type_lds(c);
-            else
+                cx[r].nltype = 0;
+            } else {
                 cx[r].ntype = 0;
+                cx[r].nltype = 0;
+            }
             b->lds = std::max(b->lds, need);
         } else {
             cx[r].ntype = 0;
+            cx[r].nltype = 0;
         }
         pd[r].tile_begin = tile;
         pd[r].nchunks = (c->nl + chunk - 1) / chunk;
