@@ -2611,7 +2611,11 @@ LKL_API int lkl_ctx_phi_packed(const lkl_ctx* c) { return c && c->d_phi8 ? 1 : 0
 
 LKL_API int lkl_ctx_set_param_types(lkl_ctx* c, int on) {
     if (!c) { set_err("lkl_ctx_set_param_types: null context"); return -1; }
-    c->param_types = on <= 0 ? 0 : on == 1 ? 1 : 2;
+    const int level = on <= 0 ? 0 : on == 1 ? 1 : 2;
+    if (level != c->param_types) {
+        c->param_types = level;
+        c->version++;  // batches built on this context refresh their views
+    }
     return 0;
 }
 

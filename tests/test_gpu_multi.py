@@ -64,6 +64,12 @@ def test_multi_matches_single(batch):
     # a re-tiled context is picked up by the batch
     ctxs[0].set_tiles(6.0, 128, 0)
     assert np.array_equal(m.score(allp), ref)
+    # so is a parameter-type level: pair-type tables, receptor tables, none
+    for level in (1, 0, 2):
+        say("multi parameter-type level", level)
+        for ctx in ctxs:
+            ctx.set_param_types(level)
+        assert np.array_equal(m.score(allp), ref), level
     m.close()
 
 
