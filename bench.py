@@ -33,7 +33,7 @@ METRIC = "pose-evals/s (dna, 12.6k-atom rec); surrogate/plain speedup at equal t
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = "profiles/score_kernel_counters.json"
 ALL_CONFIGS = ("c2-dna", "c2-pydock", "c2-vdw", "c3-cpydock", "c4")
-TIME_EVERY = 4  # kernel events on every 4th timed call
+TIME_EVERY = 8  # kernel events on every 8th timed call
 
 
 def log(*a):
